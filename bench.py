@@ -1,0 +1,221 @@
+"""Throughput of the fused per-step lattice update on one MI355X.
+
+Default workload (the headline D3Q19 256³ configuration): BGK, fp32,
+256³ channel with a solid sphere (radius 24 cells, centre x=64, y=z=128) as a
+full-way bounce-back mask, equilibrium inlet u=0.05 at x=0, zero-gradient
+outlet, periodic in y/z, seeded 1e-3 velocity perturbation.  One step = one
+fused HIP launch (pull-stream + inlet/outlet + bounce-back + collide).
+
+Prints ONE JSON line.  ``value`` is MLUPS (lattice cells updated per second,
+1e6) with the populations resident in HBM.  ``roofline`` prices the
+algorithmic 2·Q·itemsize bytes per cell (152 B for D3Q19 fp32) against the
+8 TB/s HBM peak using the average kernel duration from HIP events on the
+launch stream.  ``forms`` reports the other storage/collision forms of the
+same grid; ``cpu_baseline`` times the C oracle on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS & % HBM roofline, D3Q19 256³: plain FP32 vs FP16-store+MLP closure"
+HBM_PEAK_GBS = 8000.0
+SEED = 20260819
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=400)
+    p.add_argument("--warmup", type=int, default=40)
+    p.add_argument("--n", type=int, default=256, help="cubic grid edge")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-forms", action="store_true", help="only the headline form")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--copy-probe", action="store_true", help="also time a plain device copy of the same bytes")
+    return p.parse_args()
+
+
+def channel_case(n: int, dev, torch):
+    """Sphere mask and initial populations of the channel workload (built on the GPU)."""
+    import tensorlbm_amd as tl
+
+    g = torch.Generator(device="cpu").manual_seed(SEED)
+    a = torch.arange(n, dtype=torch.float32, device=dev)
+    z, y, x = torch.meshgrid(a, a, a, indexing="ij")
+    r = 24.0 * n / 256.0
+    solid = ((x - 64.0 * n / 256.0) ** 2 + (y - n / 2) ** 2 + (z - n / 2) ** 2) <= r * r
+    del x, y, z
+    pert = ((torch.rand((3, n, n, n), generator=g) * 2 - 1) * 1e-3).to(dev)
+    ux = 0.05 + pert[0]
+    ux[solid] = 0.0
+    rho = torch.ones((n, n, n), device=dev)
+    f = tl.equilibrium3d(rho, ux, pert[1], pert[2])
+    del pert, ux, rho
+    return solid.to(torch.int8).contiguous(), f
+
+
+def time_steps(step, f, steps, warmup, torch):
+    """Average ms per step with HIP events on torch's current stream (the launch stream)."""
+    for _ in range(warmup):
+        f = step(f)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        f = step(f)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(stop) / steps, wall, f
+
+
+def cpu_baseline(n: int, tau: float, seconds: float):
+    """C oracle (OpenMP, fp32) fused BGK step with the sphere bounce-back on the full grid."""
+    import numpy as np
+
+    from oracle import cfast, ref
+
+    if not cfast.available():
+        cfast.build()
+    rng = np.random.default_rng(SEED)
+    a = np.arange(n, dtype=np.float32)
+    r = 24.0 * n / 256.0
+    solid = ref.sphere_mask(n, n, n, 64.0 * n / 256.0, n / 2, n / 2, r)
+    pert = rng.uniform(-1e-3, 1e-3, (3, n, n, n)).astype(np.float32)
+    ux = (0.05 + pert[0]).astype(np.float32)
+    ux[solid] = 0.0
+    f = ref.equilibrium(ref.D3Q19, np.ones((n, n, n), np.float32), ux, pert[1], pert[2]).astype(np.float32)
+    del pert, ux, a
+    out = np.empty_like(f)
+    cfast.fused_bgk(f, tau, solid, out=out)  # warm-up (page faults)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        cfast.fused_bgk(f, tau, solid, out=out)
+        f, out = out, f
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or steps >= 200:
+            break
+    mlups = steps * n**3 / el / 1e6
+    return {
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "cores": cfast.num_threads(),
+        "kind": "port",
+        "sample": f"{steps} steps of the C oracle fused D3Q19 BGK fp32 step with the sphere bounce-back on the "
+                  f"full {n}^3 grid (periodic; inlet/outlet planes omitted) in {el:.1f} s on "
+                  f"{cfast.num_threads()} OpenMP threads, {platform.processor() or platform.machine()}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the per-step update is single-GPU here; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import tensorlbm_amd as tl
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    n = args.n
+    cells = n**3
+    u_in = 0.05
+    tau = 0.572  # Re = u·D/ν = 0.05·48/0.024 = 100
+    solid, f0 = channel_case(n, dev, torch)
+    q = 19
+
+    def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True):
+        f = f0.to(dtype)
+        if deviation:
+            f = (f0 - tl.lattice.D3Q19.W_EXACT64.to(dev).view(19, 1, 1, 1)).to(dtype)
+        solver = tl.FusedSolver3D(n, n, n, tau=tau, device=dev, dtype=dtype, collision=collision,
+                                  deviation=deviation, solid=solid if mask else None,
+                                  boundary="channel_x" if boundary else None, u_bc=(u_in, 0.0, 0.0))
+        ms, wall, f = time_steps(solver.step, f, args.steps, args.warmup, torch)
+        bpc = 2 * q * torch.tensor([], dtype=dtype).element_size()
+        gbs = bpc * cells / (ms * 1e-3) / 1e9
+        ok = bool(torch.isfinite(f[:, ::16, ::16, ::16].float()).all())
+        del solver, f
+        return {"ms_per_step": round(ms, 5), "mlups": round(cells / (ms * 1e-3) / 1e6, 1),
+                "bytes_per_cell": bpc, "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "finite": ok, "wall_s": round(wall, 3)}
+
+    head = form(torch.float32)
+    forms = {}
+    if not args.no_forms:
+        forms["bgk_fp32_periodic_nomask"] = form(torch.float32, boundary=False, mask=False)
+        forms["mrt_fp32_channel_sphere"] = form(torch.float32, collision="MRT")
+        forms["bgk_fp16dev_channel_sphere"] = form(torch.float16, deviation=True)
+        forms["bgk_bf16dev_channel_sphere"] = form(torch.bfloat16, deviation=True)
+        forms["bgk_fp16dev_periodic_nomask"] = form(torch.float16, deviation=True, boundary=False, mask=False)
+    if args.copy_probe:
+        src = f0.clone()
+        dst = torch.empty_like(src)
+        ms, _, _ = time_steps(lambda t: dst.copy_(src), src, args.steps, args.warmup, torch)
+        forms["device_copy_probe"] = {"ms": round(ms, 5), "bytes": 2 * src.numel() * 4,
+                                      "GBs": round(2 * src.numel() * 4 / (ms * 1e-3) / 1e9, 1)}
+        del src, dst
+    del f0
+    torch.cuda.empty_cache()
+
+    ms = head["ms_per_step"]
+    value = head["mlups"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"D3Q19 BGK fp32 {n}^3 channel, sphere r=24 at x=64 (full-way bounce-back), "
+                        "equilibrium inlet u=0.05 / copy outlet in x, periodic y/z, seeded 1e-3 perturbation",
+            "grid": [n, n, n],
+            "tau": tau,
+            "lattice": "D3Q19",
+            "collision": "BGK",
+            "storage": "fp32",
+            "solid_cells": int(solid.sum().item()),
+            "seed": SEED,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_cell": head["bytes_per_cell"],
+            "kernel": "k_lbm<D3Q19, fp32, BGK, stream, features>",
+        },
+        "forms": forms,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, tau, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

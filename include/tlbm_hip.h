@@ -1,0 +1,147 @@
+/*
+ * tlbm_hip.h — C ABI of libtlbm_hip.so, the MI355X (gfx950) per-step lattice
+ * update of TensorLBM.
+ *
+ * Conventions shared by every entry point:
+ *   - Populations are SoA ``f[Q][nz][ny][nx]``, x fastest, contiguous (the
+ *     reference layout, src/tensorlbm/d3q19.py:95-103).  D2Q9 fields
+ *     ``f[9][ny][nx]`` are passed with nz = 1.
+ *   - All pointers are device pointers owned by the caller; the library owns
+ *     no device memory.  Work is enqueued on ``stream`` (a hipStream_t; 0 is
+ *     the null stream) and nothing synchronises.
+ *   - Streaming entry points require ``f_in != f_out``: a pull step reads
+ *     neighbouring cells, so aliased buffers race across workgroups
+ *     (reference src/tensorlbm/triton_fused.py:697-701).
+ *   - Return value: TLBM_OK or a negative TLBM_ERR_* code;
+ *     tlbm_last_error() returns a message for the calling thread.
+ */
+#ifndef TLBM_HIP_H
+#define TLBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
+
+/* status codes */
+#define TLBM_OK 0
+#define TLBM_ERR_INVALID (-1)     /* bad argument value */
+#define TLBM_ERR_SHAPE (-2)       /* inconsistent or oversized shape */
+#define TLBM_ERR_DTYPE (-3)       /* unsupported storage / dtype */
+#define TLBM_ERR_UNSUPPORTED (-4) /* lattice x collision combination not built */
+#define TLBM_ERR_ALIAS (-5)       /* f_in == f_out for a streaming op */
+#define TLBM_ERR_HIP (-100)       /* HIP runtime error (launch, symbol, ...) */
+
+/* lattices (value = Q) */
+#define TLBM_D2Q9 9
+#define TLBM_D3Q19 19
+#define TLBM_D3Q27 27
+
+/* population storage */
+#define TLBM_STORE_F32 0
+#define TLBM_STORE_F64 1
+#define TLBM_STORE_F16 2      /* plain f in binary16, fp32 arithmetic */
+#define TLBM_STORE_BF16 3     /* plain f in bfloat16, fp32 arithmetic */
+#define TLBM_STORE_F16_DEV 4  /* f_i - w_i in binary16, fp32 arithmetic */
+#define TLBM_STORE_BF16_DEV 5 /* f_i - w_i in bfloat16, fp32 arithmetic */
+
+/* collision families */
+#define TLBM_COLL_NONE 0     /* stream only */
+#define TLBM_COLL_BGK 1      /* solver.py:67-71, solver3d.py:100-104, d3q27.py:423-435 */
+#define TLBM_COLL_MRT 2      /* solver.py:249-335, solver3d.py:107-160, d3q27.py:643-726 */
+#define TLBM_COLL_CM 3       /* cascaded central moments, cascaded_collision.py:445-509 */
+#define TLBM_COLL_CUMULANT 4 /* D3Q19 Hermite-regularised, cumulant.py:1159-1377 */
+#define TLBM_COLL_GEIER 5    /* D3Q27 Geier 2015 cumulants, cumulant.py:1050-1156 */
+
+/* source of the shear relaxation rate */
+#define TLBM_TAU_CONST 0 /* omega = 1/tau */
+#define TLBM_TAU_SMAG 1  /* clamped Hou closure, turbulence.py:183-208 */
+#define TLBM_TAU_FIELD 2 /* per-cell tau_eff array, triton_fused_obstacle.py:1300-1312 */
+#define TLBM_TAU_SMAG_RAW 3 /* unclamped Hou closure of cumulant.py:1316-1321 */
+#define TLBM_TAU_MLP 4   /* learned eddy-viscosity closure (tlbm_mlp_params) */
+
+/* boundary kinds for tlbm_step */
+#define TLBM_BC_NONE 0
+#define TLBM_BC_CHANNEL_X 1 /* equilibrium inlet x=0 (rho from x=1), copy outlet x=nx-1,
+                               boundaries3d.py:407-443 */
+#define TLBM_BC_LID_Y 2     /* Zou-He moving lid on y=ny-1, boundaries3d.py:324-404 */
+
+typedef struct tlbm_collision {
+  int32_t kind;         /* TLBM_COLL_* */
+  int32_t tau_mode;     /* TLBM_TAU_* */
+  int32_t eq_rho_floor; /* 1: f_eq built with max(rho,1e-12) (fused kernels); 0: with rho */
+  int32_t reserved;
+  double tau;           /* molecular relaxation time */
+  double cs_delta_sq;   /* Cs^2 Delta^2 for the Smagorinsky modes */
+  double rate[8];       /* MRT: s_e, s_eps, s_q, s_pi | CM: s_bulk, s_3, s_4, s_5, s_6 |
+                           CUMULANT/GEIER: omega_b, omega_odd, omega_even */
+  const float* tau_eff; /* TLBM_TAU_FIELD: (nz,ny,nx) float32 */
+  const void* mlp;      /* TLBM_TAU_MLP: device pointer to packed MLP parameters */
+} tlbm_collision;
+
+typedef struct tlbm_boundary {
+  int32_t kind;   /* TLBM_BC_* */
+  int32_t reserved;
+  double u[3];    /* inlet / lid velocity */
+} tlbm_boundary;
+
+/* library */
+int tlbm_version(void);
+const char* tlbm_last_error(void);
+const char* tlbm_status_string(int status);
+
+/*
+ * Fused pull-stream + boundary + collide, one launch
+ *   (replaces triton_fused()          src/tensorlbm/triton_fused.py:618-667,
+ *             triton_fused_obstacle_xfar_les()  triton_fused_obstacle.py:1695-2053).
+ * solid  : optional int8 (nz,ny,nx) mask, non-zero = solid; full-way bounce-back.
+ * bc     : optional boundary (NULL = periodic).
+ * force  : optional 3 doubles on the device; the Ladd force of the post-stream,
+ *          pre-bounce-back populations on solid cells is ADDED (caller zeroes).
+ */
+int tlbm_step(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
+              double* force, tlbm_stream_t stream);
+
+/* periodic pull stream (solver3d.py:210-239 stream3d, solver.py:396-417, d3q27.py:766-791) */
+int tlbm_stream(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+                tlbm_stream_t stream);
+
+/* local collision, f_out may equal f_in (collide_bgk3d, collide_mrt3d, ...) */
+int tlbm_collide(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+                 const tlbm_collision* coll, tlbm_stream_t stream);
+
+/* equilibrium (d2q9.py:39-58, d3q19.py:83-125, d3q27.py:353-392); dtype F32 or F64;
+ * uz may be NULL (D2Q9). */
+int tlbm_equilibrium(int lattice, int dtype, const void* rho, const void* ux, const void* uy,
+                     const void* uz, void* f_out, int64_t ncells, tlbm_stream_t stream);
+
+/* macroscopic moments (d2q9.py:61-74, d3q19.py:165-186, d3q27.py:395-420); uz may be NULL. */
+int tlbm_macroscopic(int lattice, int dtype, const void* f, void* rho, void* ux, void* uy, void* uz,
+                     int64_t ncells, tlbm_stream_t stream);
+
+/* full-way swap where(mask, f[opp], f) (boundaries3d.py:112-132); f_out may equal f_in */
+int tlbm_bounce_back(int lattice, int dtype, const void* f_in, void* f_out, const int8_t* mask,
+                     int64_t ncells, tlbm_stream_t stream);
+
+/* ADD to out[0..3] (device doubles): sum f, sum c_x f, sum c_y f, sum c_z f (FP64 wave
+ * reductions).  Works on every storage (deviation storages add back w). */
+int tlbm_moment_sums(int lattice, int store, const void* f, int64_t ncells, double* out,
+                     tlbm_stream_t stream);
+
+/* ADD the Ladd force 2 sum_solid sum_q c_q f to out[0..3] (obstacles.py:91-130), FP64. */
+int tlbm_obstacle_force(int lattice, int store, const void* f, const int8_t* solid,
+                        int64_t ncells, double* out, tlbm_stream_t stream);
+
+/* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
+int tlbm_scale(int lattice, int store, void* f, int64_t ncells, const double* scale,
+               tlbm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TLBM_HIP_H */

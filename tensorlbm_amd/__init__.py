@@ -1,0 +1,44 @@
+"""tensorlbm_amd — TensorLBM's per-step lattice update, native on MI355X (gfx950).
+
+The public names mirror the reference package ``tensorlbm`` for the hot path
+(equilibrium / macroscopic / collide_* / stream*, bounce-back, Smagorinsky,
+obstacle force, the fused step and its solver object).  Every op runs a
+hand-written HIP kernel from ``libtlbm_hip.so`` through its C ABI
+(``include/tlbm_hip.h``); there is no CPU or eager fallback.
+"""
+
+from __future__ import annotations
+
+import torch  # noqa: F401  (loads torch's HIP runtime before the library binds to it)
+
+from . import lattice
+from .fused import (
+    FusedSolver3D,
+    fused_obstacle_step,
+    fused_step,
+    hip_collide,
+    hip_fused,
+    hip_stream,
+    is_available,
+)
+from .ops import *  # noqa: F401,F403
+from .ops import __all__ as _ops_all
+from .precision import PrecisionPolicy, cast_to_compute, cast_to_store
+from .reporters import CallbackReporter, StepContext, ThroughputReporter
+
+# reference module names for the fused GPU path (src/tensorlbm/triton_fused.py,
+# src/tensorlbm/triton_fused_obstacle.py); here they run the HIP kernels
+triton_fused = hip_fused
+triton_stream = hip_stream
+triton_collide = hip_collide
+triton_fused_obstacle_xfar_les = fused_obstacle_step
+TritonFusedSolver3D = FusedSolver3D
+
+__version__ = "0.1.0"
+
+__all__ = list(_ops_all) + [
+    "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
+    "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
+    "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",
+    "TritonFusedSolver3D",
+]

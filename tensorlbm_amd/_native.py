@@ -1,0 +1,190 @@
+"""ctypes binding of ``libtlbm_hip.so`` (declared in ``include/tlbm_hip.h``).
+
+The library is loaded lazily on first use, after ``torch`` so that it binds to
+the HIP runtime torch already holds.  There is no fallback: if the shared
+object is missing or a call fails, an exception names the problem.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch
+
+LIB_NAME = "libtlbm_hip.so"
+LIB_PATH = Path(__file__).with_name(LIB_NAME)
+
+# status codes (tlbm_hip.h)
+OK = 0
+ERR_INVALID = -1
+ERR_SHAPE = -2
+ERR_DTYPE = -3
+ERR_UNSUPPORTED = -4
+ERR_ALIAS = -5
+ERR_HIP = -100
+
+# lattices / storage / collision / tau modes / boundaries
+D2Q9, D3Q19, D3Q27 = 9, 19, 27
+STORE_F32, STORE_F64, STORE_F16, STORE_BF16, STORE_F16_DEV, STORE_BF16_DEV = range(6)
+COLL_NONE, COLL_BGK, COLL_MRT, COLL_CM, COLL_CUMULANT, COLL_GEIER = range(6)
+TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP = range(5)
+BC_NONE, BC_CHANNEL_X, BC_LID_Y = range(3)
+
+
+class Collision(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("tau_mode", ctypes.c_int32),
+        ("eq_rho_floor", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("tau", ctypes.c_double),
+        ("cs_delta_sq", ctypes.c_double),
+        ("rate", ctypes.c_double * 8),
+        ("tau_eff", ctypes.c_void_p),
+        ("mlp", ctypes.c_void_p),
+    ]
+
+
+class Boundary(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("u", ctypes.c_double * 3),
+    ]
+
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+
+_SIGNATURES = {
+    "tlbm_version": ([], _I),
+    "tlbm_last_error": ([], ctypes.c_char_p),
+    "tlbm_status_string": ([_I], ctypes.c_char_p),
+    "tlbm_step": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP,
+                   ctypes.POINTER(Boundary), _VP, _VP], _I),
+    "tlbm_stream": ([_I, _I, _VP, _VP, _I, _I, _I, _VP], _I),
+    "tlbm_collide": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP], _I),
+    "tlbm_equilibrium": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I64, _VP], _I),
+    "tlbm_macroscopic": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I64, _VP], _I),
+    "tlbm_bounce_back": ([_I, _I, _VP, _VP, _VP, _I64, _VP], _I),
+    "tlbm_moment_sums": ([_I, _I, _VP, _I64, _VP, _VP], _I),
+    "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I64, _VP, _VP], _I),
+    "tlbm_scale": ([_I, _I, _VP, _I64, _VP, _VP], _I),
+}
+
+_lib: ctypes.CDLL | None = None
+
+
+def library_path() -> Path:
+    override = os.environ.get("TLBM_HIP_LIBRARY")
+    return Path(override) if override else LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the HIP library; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not path.is_file():
+            raise RuntimeError(
+                f"{LIB_NAME} not found at {path}; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` "
+                "(there is no CPU fallback)"
+            )
+        handle = ctypes.CDLL(str(path))
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = handle
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGNATURES)
+
+
+def check(rc: int, what: str) -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == OK:
+        return
+    msg = lib().tlbm_last_error().decode(errors="replace") or lib().tlbm_status_string(rc).decode()
+    text = f"{what}: {msg}"
+    if rc in (ERR_INVALID, ERR_SHAPE, ERR_DTYPE, ERR_ALIAS):
+        raise ValueError(text)
+    if rc == ERR_UNSUPPORTED:
+        raise NotImplementedError(text)
+    raise RuntimeError(text)
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(t: torch.Tensor) -> int:
+    """Raw hipStream_t of torch's current stream on the tensor's device."""
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def require_gpu(t: torch.Tensor, what: str) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise RuntimeError(f"{what} must live on a GPU device (got {t.device}); the HIP path has no CPU fallback")
+
+
+_STORE_OF_DTYPE = {
+    torch.float32: STORE_F32,
+    torch.float64: STORE_F64,
+    torch.float16: STORE_F16,
+    torch.bfloat16: STORE_BF16,
+}
+
+
+def store_code(dtype: torch.dtype, deviation: bool = False) -> int:
+    try:
+        code = _STORE_OF_DTYPE[dtype]
+    except KeyError:
+        raise ValueError(f"Unsupported dtype {dtype}; use float32, float64, float16 or bfloat16") from None
+    if deviation:
+        if code == STORE_F16:
+            return STORE_F16_DEV
+        if code == STORE_BF16:
+            return STORE_BF16_DEV
+        raise ValueError(f"deviation storage needs float16 or bfloat16, got {dtype}")
+    return code
+
+
+def make_collision(
+    kind: int,
+    tau: float,
+    *,
+    tau_mode: int = TAU_CONST,
+    eq_rho_floor: bool = True,
+    cs_delta_sq: float = 0.0,
+    rates: tuple | list = (),
+    tau_eff: torch.Tensor | None = None,
+    mlp: int | None = None,
+) -> Collision:
+    c = Collision()
+    c.kind = kind
+    c.tau_mode = tau_mode
+    c.eq_rho_floor = 1 if eq_rho_floor else 0
+    c.tau = float(tau)
+    c.cs_delta_sq = float(cs_delta_sq)
+    for i, r in enumerate(rates):
+        c.rate[i] = float(r)
+    c.tau_eff = ptr(tau_eff)
+    c.mlp = mlp
+    return c
+
+
+def make_boundary(kind: int = BC_NONE, u=(0.0, 0.0, 0.0)) -> Boundary:
+    b = Boundary()
+    b.kind = kind
+    for i in range(3):
+        b.u[i] = float(u[i])
+    return b

@@ -1,0 +1,339 @@
+// Register-resident collision operators, templated on the velocity set L and
+// the arithmetic type C (float or double).  Each operator maps the Q
+// post-stream populations f[] of one cell to their post-collision values in
+// place.  Lattice constants and moment tables are compile-time, so unrolled
+// loops fold every zero / unit coefficient.
+#pragma once
+
+#include "lattice.hpp"
+#include "tables.inc"
+
+namespace tlbm {
+
+template <typename C>
+struct Macro {
+  C rho, rho_safe, ux, uy, uz;
+};
+
+// rho = sum f, u = sum c f / max(rho, 1e-12)   (d3q19.py:165-186)
+template <class L, typename C>
+__device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
+  C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    rho += f[q];
+    if (L::cx[q] != 0) jx += C(L::cx[q]) * f[q];
+    if (L::cy[q] != 0) jy += C(L::cy[q]) * f[q];
+    if (L::cz[q] != 0) jz += C(L::cz[q]) * f[q];
+  }
+  Macro<C> m;
+  m.rho = rho;
+  m.rho_safe = rho > C(1e-12) ? rho : C(1e-12);
+  const C inv = C(1) / m.rho_safe;
+  m.ux = jx * inv;
+  m.uy = jy * inv;
+  m.uz = jz * inv;
+  return m;
+}
+
+// w rho (1 + 3 cu + 4.5 cu^2 - 1.5 u^2)   (d3q19.py:119-121)
+template <class L, typename C>
+__device__ __forceinline__ void equilibrium(C rho, C ux, C uy, C uz, C (&feq)[L::Q]) {
+  const C usq15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    C cu = C(0);
+    if (L::cx[q] != 0) cu += C(L::cx[q]) * ux;
+    if (L::cy[q] != 0) cu += C(L::cy[q]) * uy;
+    if (L::cz[q] != 0) cu += C(L::cz[q]) * uz;
+    feq[q] = C(L::w[q]) * rho * (C(1) + C(3) * cu + C(4.5) * cu * cu - usq15);
+  }
+}
+
+// Second-order non-equilibrium stress Pi_ab = sum c_a c_b fneq (xx yy zz xy xz yz)
+template <class L, typename C>
+__device__ __forceinline__ void neq_stress(const C (&fneq)[L::Q], C (&pi)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) pi[k] = C(0);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    if (L::cx[q] != 0) pi[0] += fneq[q];
+    if (L::cy[q] != 0) pi[1] += fneq[q];
+    if (L::cz[q] != 0) pi[2] += fneq[q];
+    if (L::cx[q] * L::cy[q] != 0) pi[3] += C(L::cx[q] * L::cy[q]) * fneq[q];
+    if (L::cx[q] * L::cz[q] != 0) pi[4] += C(L::cx[q] * L::cz[q]) * fneq[q];
+    if (L::cy[q] * L::cz[q] != 0) pi[5] += C(L::cy[q] * L::cz[q]) * fneq[q];
+  }
+}
+
+template <typename C>
+__device__ __forceinline__ C pi_norm(const C (&pi)[6]) {
+  return sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+              C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
+}
+
+// Hou et al. tau_eff clamped to [0.5001, max(1, tau+0.5)]  (turbulence.py:183-208)
+template <typename C>
+__device__ __forceinline__ C smagorinsky_omega(C pn, C rho_safe, C tau, C cs_delta_sq, C tau_cap) {
+  C disc = tau * tau + C(18) * cs_delta_sq * pn / rho_safe;
+  disc = disc > C(0) ? disc : C(0);
+  C te = C(0.5) * (tau + sqrt(disc));
+  te = te < C(0.5001) ? C(0.5001) : te;
+  te = te > tau_cap ? tau_cap : te;
+  return C(1) / te;
+}
+
+// Unclamped closure of collide_cumulant_d3q19 (cumulant.py:1316-1321)
+template <typename C>
+__device__ __forceinline__ C smagorinsky_raw_omega(C pn, C rho_safe, C tau, C cs_delta_sq) {
+  return C(2) / (tau + sqrt(tau * tau + C(18) * cs_delta_sq * pn / rho_safe));
+}
+
+// 4.5 w_q (H_q : Pi) with doubled off-diagonals (cumulant.py:1331-1362)
+template <class L, typename C>
+__device__ __forceinline__ C hermite2(int q, const C (&pi)[6]) {
+  const C t = C(1.0 / 3.0);
+  const C h = (C(L::cx[q] * L::cx[q]) - t) * pi[0] + (C(L::cy[q] * L::cy[q]) - t) * pi[1] +
+              (C(L::cz[q] * L::cz[q]) - t) * pi[2] + C(2 * L::cx[q] * L::cy[q]) * pi[3] +
+              C(2 * L::cx[q] * L::cz[q]) * pi[4] + C(2 * L::cy[q] * L::cz[q]) * pi[5];
+  return C(4.5) * C(L::w[q]) * h;
+}
+
+template <class L>
+struct MrtTab;
+template <>
+struct MrtTab<D2Q9> {
+  static constexpr const double (&M)[9][9] = tables::MRT9;
+  static constexpr const double (&Mi)[9][9] = tables::MRT9_INV;
+};
+template <>
+struct MrtTab<D3Q19> {
+  static constexpr const double (&M)[19][19] = tables::MRT19;
+  static constexpr const double (&Mi)[19][19] = tables::MRT19_INV;
+};
+template <>
+struct MrtTab<D3Q27> {
+  static constexpr const double (&M)[27][27] = tables::MRT27;
+  static constexpr const double (&Mi)[27][27] = tables::MRT27_INV;
+};
+
+// f* = f - M^-1 S M (f - f_eq); S holds the per-moment rates, with the shear
+// entries given by omega (may vary per cell).  solver3d.py:107-160.
+template <class L, typename C>
+__device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C* s, C omega) {
+  constexpr int Q = L::Q;
+  C m[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    C acc = C(0);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      constexpr double zero = 0.0;
+      if (MrtTab<L>::M[i][j] != zero) acc += C(MrtTab<L>::M[i][j]) * (f[j] - feq[j]);
+    }
+    m[i] = acc;
+  }
+  // rate per moment: s[i] < 0 marks "shear" (use omega)
+#pragma unroll
+  for (int i = 0; i < Q; ++i) m[i] *= (s[i] < C(0) ? omega : s[i]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    C acc = C(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      if (MrtTab<L>::Mi[q][i] != 0.0) acc += C(MrtTab<L>::Mi[q][i]) * m[i];
+    }
+    f[q] -= acc;
+  }
+}
+
+template <class L>
+struct CmTab;
+template <>
+struct CmTab<D3Q19> {
+  static constexpr const double (&M)[19][19] = tables::CM19;
+  static constexpr const double (&Mi)[19][19] = tables::CM19_INV;
+  static constexpr const int (&SX)[5][3] = tables::CM19_SHIFT_X;
+  static constexpr const int (&SY)[5][3] = tables::CM19_SHIFT_Y;
+  static constexpr const int (&SZ)[5][3] = tables::CM19_SHIFT_Z;
+  static constexpr int NG = 5;
+};
+template <>
+struct CmTab<D3Q27> {
+  static constexpr const double (&M)[27][27] = tables::CM27;
+  static constexpr const double (&Mi)[27][27] = tables::CM27_INV;
+  static constexpr const int (&SX)[9][3] = tables::CM27_SHIFT_X;
+  static constexpr const int (&SY)[9][3] = tables::CM27_SHIFT_Y;
+  static constexpr const int (&SZ)[9][3] = tables::CM27_SHIFT_Z;
+  static constexpr int NG = 9;
+};
+
+// binomial 1-D shift, sign -1 forward (raw -> central), +1 inverse
+template <int NG, typename C, int Q>
+__device__ __forceinline__ void shift1d(C (&m)[Q], const int (&g)[NG][3], C u, C sign) {
+#pragma unroll
+  for (int k = 0; k < NG; ++k) {
+    const C m0 = m[g[k][0]], m1 = m[g[k][1]], m2 = m[g[k][2]];
+    m[g[k][1]] = m1 + sign * u * m0;
+    m[g[k][2]] = m2 + sign * C(2) * u * m1 + u * u * m0;
+  }
+}
+
+template <class L, typename C>
+__device__ __forceinline__ void moments(const C (&v)[L::Q], C (&m)[L::Q]) {
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    C acc = C(0);
+#pragma unroll
+    for (int j = 0; j < L::Q; ++j)
+      if (CmTab<L>::M[i][j] != 0.0) acc += C(CmTab<L>::M[i][j]) * v[j];
+    m[i] = acc;
+  }
+}
+
+template <class L, typename C>
+__device__ __forceinline__ void populations(const C (&m)[L::Q], C (&v)[L::Q]) {
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    C acc = C(0);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i)
+      if (CmTab<L>::Mi[q][i] != 0.0) acc += C(CmTab<L>::Mi[q][i]) * m[i];
+    v[q] = acc;
+  }
+}
+
+template <class L, typename C>
+__device__ __forceinline__ void to_central(C (&m)[L::Q], C ux, C uy, C uz) {
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SX, ux, C(-1));
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SY, uy, C(-1));
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(-1));
+}
+
+template <class L, typename C>
+__device__ __forceinline__ void to_raw(C (&m)[L::Q], C ux, C uy, C uz) {
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(1));
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SY, uy, C(1));
+  shift1d<CmTab<L>::NG>(m, CmTab<L>::SX, ux, C(1));
+}
+
+// Cascaded central-moment collision on f_neq (cascaded_collision.py:445-509).
+// rate = {s_bulk, s_3, s_4, s_5, s_6}.
+template <class L, typename C>
+__device__ __forceinline__ void cascaded(C (&f)[L::Q], const C (&feq)[L::Q], const Macro<C>& mc,
+                                         C omega, const C* rate) {
+  constexpr int Q = L::Q;
+  C v[Q], m[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = f[q] - feq[q];
+  moments<L>(v, m);
+  to_central<L>(m, mc.ux, mc.uy, mc.uz);
+  const C tr = m[4] + m[5] + m[6];
+  const C trs = (C(1) - rate[0]) * tr;
+  const C third = C(1.0 / 3.0);
+#pragma unroll
+  for (int i = 4; i < 7; ++i) m[i] = (C(1) - omega) * (m[i] - tr * third) + trs * third;
+#pragma unroll
+  for (int i = 7; i < 10; ++i) m[i] = (C(1) - omega) * m[i];
+  if (Q == 19) {
+#pragma unroll
+    for (int i = 10; i < 16; ++i) m[i] = (C(1) - rate[1]) * m[i];
+#pragma unroll
+    for (int i = 16; i < Q; ++i) m[i] = (C(1) - rate[2]) * m[i];
+  } else {
+#pragma unroll
+    for (int i = 10; i < 17; ++i) m[i] = (C(1) - rate[1]) * m[i];
+#pragma unroll
+    for (int i = 17; i < 23; ++i) m[i] = (C(1) - rate[2]) * m[i];
+#pragma unroll
+    for (int i = 23; i < 26; ++i) m[i] = (C(1) - rate[3]) * m[i];
+#pragma unroll
+    for (int i = 26; i < Q; ++i) m[i] = (C(1) - rate[4]) * m[i];
+  }
+  to_raw<L>(m, mc.ux, mc.uy, mc.uz);
+  populations<L>(m, v);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = feq[q] + v[q];
+}
+
+// D3Q19 Hermite-regularised cumulant (cumulant.py:1159-1377).
+// rate = {omega_b, omega_odd (unused), omega_even}.
+template <class L, typename C>
+__device__ __forceinline__ void cumulant19(C (&f)[L::Q], const C (&feq)[L::Q], const C (&pi)[6],
+                                           C omega, const C* rate) {
+  const C ob = rate[0], oe = rate[2];
+  const C tr = pi[0] + pi[1] + pi[2];
+  const C db = (ob - omega) * tr / C(3);
+  C ps[6];
+  ps[0] = pi[0] - omega * pi[0] - db;
+  ps[1] = pi[1] - omega * pi[1] - db;
+  ps[2] = pi[2] - omega * pi[2] - db;
+  ps[3] = pi[3] - omega * pi[3];
+  ps[4] = pi[4] - omega * pi[4];
+  ps[5] = pi[5] - omega * pi[5];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const C fneq = f[q] - feq[q];
+    const C freg = hermite2<L>(q, ps);
+    const C fho = fneq - hermite2<L>(q, pi);
+    f[q] = feq[q] + freg + (C(1) - oe) * fho;
+  }
+}
+
+// Geier 2015 cumulants on the full distribution, relaxed towards the
+// equilibrium's cumulants (cumulant.py:1050-1156).  rate = {omega_b,
+// omega_odd, omega_even}.
+template <typename C>
+__device__ __forceinline__ void geier27(C (&f)[27], const C (&feq)[27], const Macro<C>& mc, C omega,
+                                        const C* rate) {
+  using L = D3Q27;
+  C m[27], me[27], kt[27], c[27], ce[27];
+  moments<L>(f, m);
+  moments<L>(feq, me);
+  to_central<L>(m, mc.ux, mc.uy, mc.uz);
+  to_central<L>(me, mc.ux, mc.uy, mc.uz);
+  // C = rho * C~(kappa / rho), component 0 carries rho
+  {
+    const C r = m[0], ir = C(1) / r;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) kt[i] = m[i] * ir;
+    tables::central_to_cumulant27(kt, c);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) c[i] *= r;
+    c[0] = r;
+  }
+  {
+    const C r = me[0], ir = C(1) / r;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) kt[i] = me[i] * ir;
+    tables::central_to_cumulant27(kt, ce);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) ce[i] *= r;
+    ce[0] = r;
+  }
+  const C ob = rate[0], oo = rate[1], oe = rate[2];
+  const C d4 = c[4] - ce[4], d5 = c[5] - ce[5], d6 = c[6] - ce[6];
+  const C tr3 = (d4 + d5 + d6) / C(3);
+  c[4] = c[4] - (omega * (d4 - tr3) + ob * tr3);
+  c[5] = c[5] - (omega * (d5 - tr3) + ob * tr3);
+  c[6] = c[6] - (omega * (d6 - tr3) + ob * tr3);
+#pragma unroll
+  for (int i = 7; i < 10; ++i) c[i] = c[i] - omega * (c[i] - ce[i]);
+#pragma unroll
+  for (int i = 10; i < 17; ++i) c[i] = c[i] - oo * (c[i] - ce[i]);
+#pragma unroll
+  for (int i = 17; i < 27; ++i) c[i] = c[i] - oe * (c[i] - ce[i]);
+  {
+    const C r = c[0], ir = C(1) / r;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) ce[i] = c[i] * ir;  // reuse ce as C~
+    tables::cumulant_to_central27(ce, kt);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) m[i] = kt[i] * r;
+    m[0] = r;
+  }
+  to_raw<L>(m, mc.ux, mc.uy, mc.uz);
+  populations<L>(m, f);
+}
+
+}  // namespace tlbm

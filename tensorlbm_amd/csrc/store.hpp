@@ -1,0 +1,89 @@
+// Population storage formats.  Arithmetic is float for the 32/16-bit
+// storages and double for binary64.  The deviation storages keep
+// g_i = f_i - w_i, so the 16-bit quantisation acts on the small
+// non-equilibrium part instead of the O(w_i) population.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.hpp"
+
+namespace tlbm {
+
+enum StoreKind { ST_F32 = 0, ST_F64 = 1, ST_F16 = 2, ST_BF16 = 3, ST_F16_DEV = 4, ST_BF16_DEV = 5 };
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// round to nearest even, NaN kept quiet
+__device__ __forceinline__ uint16_t f32_to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+template <int S>
+struct Store;
+
+template <>
+struct Store<ST_F32> {
+  using T = float;
+  using C = float;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int) { return v; }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int) { return v; }
+};
+
+template <>
+struct Store<ST_F64> {
+  using T = double;
+  using C = double;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int) { return v; }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int) { return v; }
+};
+
+template <>
+struct Store<ST_F16> {
+  using T = _Float16;
+  using C = float;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int) { return float(v); }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int) { return _Float16(v); }
+};
+
+template <>
+struct Store<ST_BF16> {
+  using T = uint16_t;
+  using C = float;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int) { return bf16_to_f32(v); }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int) { return f32_to_bf16(v); }
+};
+
+template <>
+struct Store<ST_F16_DEV> {
+  using T = _Float16;
+  using C = float;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int q) { return float(v) + float(L::w[q]); }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int q) { return _Float16(v - float(L::w[q])); }
+};
+
+template <>
+struct Store<ST_BF16_DEV> {
+  using T = uint16_t;
+  using C = float;
+  template <class L>
+  __device__ static __forceinline__ C get(T v, int q) { return bf16_to_f32(v) + float(L::w[q]); }
+  template <class L>
+  __device__ static __forceinline__ T put(C v, int q) { return f32_to_bf16(v - float(L::w[q])); }
+};
+
+}  // namespace tlbm

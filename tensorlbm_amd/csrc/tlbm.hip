@@ -1,0 +1,748 @@
+// libtlbm_hip: the per-step lattice update of TensorLBM for MI355X (gfx950).
+//
+// One kernel template (k_lbm) covers the fused pull-stream + boundary +
+// bounce-back + collide step, the stream-only op and the collide-only op for
+// D2Q9 / D3Q19 / D3Q27 and all storage formats.  Each thread owns one cell:
+// it gathers the Q incoming populations (coalesced along x; a +-1 x shift
+// costs one extra cache line per row, served by L2), collides in registers
+// and writes Q coalesced values at its own cell.  Full-way bounce-back at a
+// solid cell is a register lane swap f[q] <-> f[opp(q)] of the pulled values
+// (pull[opp q](x) = b[opp q][x + c_q]), so walls cost no extra loads.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <type_traits>
+
+#include "../../include/tlbm_hip.h"
+#include "collide.hpp"
+#include "store.hpp"
+
+namespace tlbm {
+
+enum Feat { FEAT_NONE = 0, FEAT_ALL = 1 };
+
+template <typename C>
+struct KArgs {
+  const void* fin;
+  void* fout;
+  const int8_t* solid;
+  const float* tau_eff;
+  const void* mlp;
+  double* force;
+  uint64_t N;
+  int nx, ny, nz, tiles_x, tiles_y;
+  int tau_mode, eq_floor, bc_kind;
+  C omega, tau, cs_delta_sq, tau_cap;
+  C rate[8];
+  C s[27];
+  C bc_u[3];
+};
+
+template <class ST, class L>
+__device__ __forceinline__ typename ST::C ld(const typename ST::T* p, int q, uint64_t N, uint32_t i) {
+  return ST::template get<L>(p[uint64_t(q) * N + i], q);
+}
+
+template <class ST, class L>
+__device__ __forceinline__ void st(typename ST::T* p, int q, uint64_t N, uint32_t i, typename ST::C v) {
+  p[uint64_t(q) * N + i] = ST::template put<L>(v, q);
+}
+
+// source cell of lane q for a pull at (x, y, z) with periodic wrap
+template <class L>
+__device__ __forceinline__ uint32_t pull_src(int q, int x, int y, int z, int nx, int ny, int nz) {
+  int sx = x - L::cx[q], sy = y - L::cy[q], sz = z - L::cz[q];
+  sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
+  sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
+  sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
+  return (uint32_t(sz) * uint32_t(ny) + uint32_t(sy)) * uint32_t(nx) + uint32_t(sx);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <class L, int S, int COLL, bool STREAM, int FEAT>
+__global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  using C = typename ST::C;
+  constexpr int Q = L::Q;
+
+  const uint32_t tile = blockIdx.x;
+  const uint32_t txi = tile % uint32_t(a.tiles_x);
+  const uint32_t rest = tile / uint32_t(a.tiles_x);
+  const uint32_t tyi = rest % uint32_t(a.tiles_y);
+  const int z = int(rest / uint32_t(a.tiles_y));
+  const int x = int(txi * blockDim.x + threadIdx.x);
+  const int y = int(tyi * blockDim.y + threadIdx.y);
+  const bool active = x < a.nx && y < a.ny;
+  constexpr bool kForcePath = (FEAT == FEAT_ALL);
+  if (!kForcePath && !active) return;
+  // inactive lanes (force path only) work on cell (0, 0, z): in bounds, never stored
+  const int xx = active ? x : 0;
+  const int yy = active ? y : 0;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint64_t N = a.N;
+  const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(yy)) * uint32_t(nx) + uint32_t(xx);
+  const T* fin = static_cast<const T*>(a.fin);
+
+  C f[Q];
+  if constexpr (STREAM) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, pull_src<L>(q, xx, yy, z, nx, ny, nz));
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, cell);
+  }
+
+  bool solid = false;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if constexpr (FEAT == FEAT_ALL) {
+    if (a.solid != nullptr) solid = active && a.solid[cell] != 0;
+    if (a.force != nullptr && solid) {
+      // Ladd sample: post-stream, pre-bounce-back (obstacles.py:91-130)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (L::cx[q] != 0) fx += double(L::cx[q]) * double(f[q]);
+        if (L::cy[q] != 0) fy += double(L::cy[q]) * double(f[q]);
+        if (L::cz[q] != 0) fz += double(L::cz[q]) * double(f[q]);
+      }
+    }
+    if (STREAM && a.bc_kind == TLBM_BC_CHANNEL_X) {
+      if (xx == 0) {
+        // equilibrium inlet, density of the pulled state at x = 1
+        C rho1 = C(0);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) rho1 += ld<ST, L>(fin, q, N, pull_src<L>(q, 1, yy, z, nx, ny, nz));
+        equilibrium<L>(rho1, a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
+      } else if (xx == nx - 1) {
+        // zero-gradient outlet: copy the pulled state of x = nx-2
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, pull_src<L>(q, nx - 2, yy, z, nx, ny, nz));
+      }
+    } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1) {
+      // Zou-He moving lid, ux = u_lid (boundaries3d.py:324-404)
+      if constexpr (Q == 19) {
+      const C s0 = f[0] + f[1] + f[2] + f[5] + f[6] + f[11] + f[12] + f[13] + f[14];
+      const C sp = f[3] + f[7] + f[10] + f[15] + f[17];
+      const C rho = s0 + C(2) * sp;
+      const C dx = rho * a.bc_u[0] - ((f[1] - f[2] + f[11] - f[12] + f[13] - f[14]) + (f[7] - f[10]));
+      const C sz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14];
+      const C h = C(0.5) * (f[7] + f[10]);
+      f[4] = f[3];
+      f[8] = h - C(0.5) * dx;
+      f[9] = h + C(0.5) * dx;
+      f[16] = f[15] + C(0.5) * sz;
+      f[18] = f[17] - C(0.5) * sz;
+      }
+    }
+    if (solid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (q < L::opp[q]) {
+          const C t = f[q];
+          f[q] = f[L::opp[q]];
+          f[L::opp[q]] = t;
+        }
+      }
+    }
+  }
+
+  if constexpr (COLL != TLBM_COLL_NONE) {
+    const Macro<C> mc = macro<L>(f);
+    C feq[Q];
+    equilibrium<L>(a.eq_floor ? mc.rho_safe : mc.rho, mc.ux, mc.uy, mc.uz, feq);
+    C omega = a.omega;
+    C pi[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+    const bool need_pi = (COLL == TLBM_COLL_CUMULANT) || a.tau_mode == TLBM_TAU_SMAG ||
+                         a.tau_mode == TLBM_TAU_SMAG_RAW;
+    if (need_pi) {
+      C fneq[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
+      neq_stress<L>(fneq, pi);
+    }
+    if (a.tau_mode == TLBM_TAU_SMAG) {
+      omega = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
+    } else if (a.tau_mode == TLBM_TAU_SMAG_RAW) {
+      omega = smagorinsky_raw_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq);
+    } else if (a.tau_mode == TLBM_TAU_FIELD) {
+      omega = C(1) / C(a.tau_eff[cell]);
+    }
+
+    if constexpr (COLL == TLBM_COLL_BGK) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = f[q] - omega * (f[q] - feq[q]);
+    } else if constexpr (COLL == TLBM_COLL_MRT) {
+      mrt<L>(f, feq, a.s, omega);
+    } else if constexpr (COLL == TLBM_COLL_CM) {
+      if constexpr (Q != 9) cascaded<L>(f, feq, mc, omega, a.rate);
+    } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
+      cumulant19<L>(f, feq, pi, omega, a.rate);
+    } else if constexpr (COLL == TLBM_COLL_GEIER) {
+      if constexpr (Q == 27) geier27<C>(f, feq, mc, omega, a.rate);
+    }
+  }
+
+  if (active) {
+    T* fout = static_cast<T*>(a.fout);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) st<ST, L>(fout, q, N, cell, f[q]);
+  }
+
+  if constexpr (FEAT == FEAT_ALL) {
+    if (a.force == nullptr) return;
+    // one FP64 atomic per wave that holds solid cells
+    if (__any(solid)) {
+      fx = wave_sum(fx);
+      fy = wave_sum(fy);
+      fz = wave_sum(fz);
+      if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) {
+        atomicAdd(a.force + 0, 2.0 * fx);
+        atomicAdd(a.force + 1, 2.0 * fy);
+        atomicAdd(a.force + 2, 2.0 * fz);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise ops
+// ---------------------------------------------------------------------------
+
+template <class L, typename T>
+__global__ __launch_bounds__(256) void k_equilibrium(const T* rho, const T* ux, const T* uy, const T* uz,
+                                                     T* out, uint64_t N) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  T feq[L::Q];
+  equilibrium<L>(rho[i], ux[i], uy[i], uz ? uz[i] : T(0), feq);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) out[uint64_t(q) * N + i] = feq[q];
+}
+
+template <class L, typename T>
+__global__ __launch_bounds__(256) void k_macroscopic(const T* f, T* rho, T* ux, T* uy, T* uz, uint64_t N) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  T v[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) v[q] = f[uint64_t(q) * N + i];
+  const Macro<T> m = macro<L>(v);
+  rho[i] = m.rho;
+  ux[i] = m.ux;
+  uy[i] = m.uy;
+  if (uz) uz[i] = m.uz;
+}
+
+template <class L, typename T>
+__global__ __launch_bounds__(256) void k_bounce_back(const T* fin, T* fout, const int8_t* mask, uint64_t N) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  T v[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) v[q] = fin[uint64_t(q) * N + i];
+  const bool s = mask[i] != 0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) fout[uint64_t(q) * N + i] = s ? v[L::opp[q]] : v[q];
+}
+
+// sums of f, c_x f, c_y f, c_z f (solid != nullptr: Ladd force over solid cells, x2)
+template <class L, int S>
+__global__ __launch_bounds__(256) void k_moment_sums(const void* fv, const int8_t* solid, uint64_t N,
+                                                     double* out) {
+  using ST = Store<S>;
+  const typename ST::T* f = static_cast<const typename ST::T*>(fv);
+  double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
+    if (solid != nullptr && solid[i] == 0) continue;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      const double v = double(ST::template get<L>(f[uint64_t(q) * N + i], q));
+      s0 += v;
+      if (L::cx[q] != 0) sx += double(L::cx[q]) * v;
+      if (L::cy[q] != 0) sy += double(L::cy[q]) * v;
+      if (L::cz[q] != 0) sz += double(L::cz[q]) * v;
+    }
+  }
+  s0 = wave_sum(s0);
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  __shared__ double part[4][4];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  if (lane == 0) {
+    part[wid][0] = s0;
+    part[wid][1] = sx;
+    part[wid][2] = sy;
+    part[wid][3] = sz;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < int(blockDim.x / 64); ++w) t += part[w][threadIdx.x];
+    const double k = (solid != nullptr && threadIdx.x > 0) ? 2.0 : 1.0;
+    if (solid != nullptr && threadIdx.x == 0) return;
+    atomicAdd(out + (solid != nullptr ? threadIdx.x - 1 : threadIdx.x), k * t);
+  }
+}
+
+template <class L, int S>
+__global__ __launch_bounds__(256) void k_scale(void* fv, uint64_t N, const double* scale) {
+  using ST = Store<S>;
+  typename ST::T* f = static_cast<typename ST::T*>(fv);
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const typename ST::C s = typename ST::C(*scale);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const uint64_t k = uint64_t(q) * N + i;
+    f[k] = ST::template put<L>(ST::template get<L>(f[k], q) * s, q);
+  }
+}
+
+}  // namespace tlbm
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using namespace tlbm;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(TLBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  return TLBM_OK;
+}
+
+bool lattice_ok(int l) { return l == TLBM_D2Q9 || l == TLBM_D3Q19 || l == TLBM_D3Q27; }
+bool store_ok(int s) { return s >= TLBM_STORE_F32 && s <= TLBM_STORE_BF16_DEV; }
+
+int check_dims(int lattice, int nz, int ny, int nx) {
+  if (!lattice_ok(lattice)) return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", lattice);
+  if (nz < 1 || ny < 1 || nx < 1) return fail(TLBM_ERR_SHAPE, "grid dims must be >= 1 (got %d,%d,%d)", nz, ny, nx);
+  if (lattice == TLBM_D2Q9 && nz != 1) return fail(TLBM_ERR_SHAPE, "D2Q9 requires nz == 1 (got %d)", nz);
+  const uint64_t n = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  if (n >= (uint64_t(1) << 31)) return fail(TLBM_ERR_SHAPE, "grid of %llu cells exceeds 2^31", (unsigned long long)n);
+  return TLBM_OK;
+}
+
+template <typename C>
+int fill_args(KArgs<C>& a, int lattice, const void* fin, void* fout, int nz, int ny, int nx,
+              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
+              int* bx_out, int* by_out, uint32_t* nblocks) {
+  memset(&a, 0, sizeof(a));
+  a.fin = fin;
+  a.fout = fout;
+  a.solid = solid;
+  a.force = force;
+  a.N = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  const int bx = nx <= 64 ? 64 : (nx <= 128 ? 128 : 256);
+  const int by = 256 / bx;
+  a.tiles_x = (nx + bx - 1) / bx;
+  a.tiles_y = (ny + by - 1) / by;
+  *bx_out = bx;
+  *by_out = by;
+  const uint64_t nb = uint64_t(a.tiles_x) * uint64_t(a.tiles_y) * uint64_t(nz);
+  if (nb >= (uint64_t(1) << 31)) return fail(TLBM_ERR_SHAPE, "too many workgroups");
+  *nblocks = uint32_t(nb);
+  a.tau_mode = TLBM_TAU_CONST;
+  a.eq_floor = 1;
+  a.omega = C(1);
+  if (coll != nullptr) {
+    if (coll->kind != TLBM_COLL_NONE && !(coll->tau > 0.5))
+      return fail(TLBM_ERR_INVALID, "tau must be > 0.5 (got %g)", coll->tau);
+    a.tau_mode = coll->tau_mode;
+    a.eq_floor = coll->eq_rho_floor;
+    a.tau = C(coll->tau);
+    a.omega = C(1.0 / coll->tau);
+    a.cs_delta_sq = C(coll->cs_delta_sq);
+    a.tau_cap = C(coll->tau + 0.5 > 1.0 ? coll->tau + 0.5 : 1.0);
+    for (int i = 0; i < 8; ++i) a.rate[i] = C(coll->rate[i]);
+    a.tau_eff = coll->tau_eff;
+    a.mlp = coll->mlp;
+    if (coll->tau_mode < TLBM_TAU_CONST || coll->tau_mode > TLBM_TAU_MLP)
+      return fail(TLBM_ERR_INVALID, "unknown tau_mode %d", coll->tau_mode);
+    if (coll->tau_mode == TLBM_TAU_FIELD && coll->tau_eff == nullptr)
+      return fail(TLBM_ERR_INVALID, "TLBM_TAU_FIELD needs tau_eff");
+    if (coll->tau_mode == TLBM_TAU_MLP)
+      return fail(TLBM_ERR_UNSUPPORTED, "TLBM_TAU_MLP is served by tlbm_step_mlp");
+    if (coll->kind == TLBM_COLL_MRT) {
+      // MRT rate vector; negative marks the shear moments (rate = omega)
+      const double se = coll->rate[0], seps = coll->rate[1], sq = coll->rate[2], spi = coll->rate[3];
+      double s[27];
+      int q = lattice;
+      for (int i = 0; i < 27; ++i) s[i] = 0.0;
+      if (lattice == TLBM_D2Q9) {
+        const double v[9] = {0, se, seps, 0, sq, 0, sq, -1, -1};
+        for (int i = 0; i < 9; ++i) s[i] = v[i];
+      } else if (lattice == TLBM_D3Q19) {
+        const double v[19] = {0, se, seps, 0, sq, 0, sq, 0, sq, -1, -1, -1, -1, -1, spi, spi, 1, 1, 1};
+        for (int i = 0; i < 19; ++i) s[i] = v[i];
+      } else {
+        for (int i = 4; i < 27; ++i) s[i] = i == 4 ? se : (i < 10 ? -1 : (i < 19 ? sq : (i == 19 ? seps : spi)));
+      }
+      for (int i = 0; i < q; ++i) a.s[i] = C(s[i]);
+    }
+  }
+  a.bc_kind = TLBM_BC_NONE;
+  if (bc != nullptr && bc->kind != TLBM_BC_NONE) {
+    if (bc->kind == TLBM_BC_CHANNEL_X) {
+      if (nx < 3) return fail(TLBM_ERR_SHAPE, "channel boundary needs nx >= 3");
+    } else if (bc->kind == TLBM_BC_LID_Y) {
+      if (lattice != TLBM_D3Q19) return fail(TLBM_ERR_UNSUPPORTED, "moving lid is D3Q19 only");
+    } else {
+      return fail(TLBM_ERR_INVALID, "unknown boundary kind %d", bc->kind);
+    }
+    a.bc_kind = bc->kind;
+    for (int i = 0; i < 3; ++i) a.bc_u[i] = C(bc->u[i]);
+  }
+  return TLBM_OK;
+}
+
+// ---- instantiation table ----------------------------------------------------
+
+template <class L, int S, int COLL, bool STREAM, int FEAT>
+int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
+  return check_launch("k_lbm");
+}
+
+template <class L, int COLL>
+constexpr bool coll_built() {
+  if (COLL == TLBM_COLL_NONE || COLL == TLBM_COLL_BGK || COLL == TLBM_COLL_MRT) return true;
+  if (COLL == TLBM_COLL_CM) return L::Q != 9;
+  if (COLL == TLBM_COLL_CUMULANT) return L::Q == 19;
+  if (COLL == TLBM_COLL_GEIER) return L::Q == 27;
+  return false;
+}
+
+template <class L, int S, int COLL>
+int run_coll(bool stream_, bool feat, int lattice, const void* fin, void* fout, int nz, int ny, int nx,
+             const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
+             hipStream_t s) {
+  if constexpr (!coll_built<L, COLL>()) {
+    return fail(TLBM_ERR_UNSUPPORTED, "collision %d is not built for Q=%d", COLL, L::Q);
+  } else {
+    KArgs<typename Store<S>::C> a;
+    int bx, by;
+    uint32_t nb;
+    int rc = fill_args(a, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, &bx, &by, &nb);
+    if (rc != TLBM_OK) return rc;
+    if (stream_) {
+      if (feat) return launch_lbm<L, S, COLL, true, FEAT_ALL>(a, bx, by, nb, s);
+      return launch_lbm<L, S, COLL, true, FEAT_NONE>(a, bx, by, nb, s);
+    }
+    if constexpr (S == ST_F32 || S == ST_F64) {
+      return launch_lbm<L, S, COLL, false, FEAT_NONE>(a, bx, by, nb, s);
+    } else {
+      return fail(TLBM_ERR_DTYPE, "collide-only ops take float32 or float64 populations");
+    }
+  }
+}
+
+template <class L, int S>
+int run_store(int kind, bool stream_, bool feat, int lattice, const void* fin, void* fout, int nz, int ny,
+              int nx, const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
+              double* force, hipStream_t s) {
+  switch (kind) {
+#define TLBM_CASE(K) \
+  case K:            \
+    return run_coll<L, S, K>(stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, s);
+    TLBM_CASE(TLBM_COLL_NONE)
+    TLBM_CASE(TLBM_COLL_BGK)
+    TLBM_CASE(TLBM_COLL_MRT)
+    TLBM_CASE(TLBM_COLL_CM)
+    TLBM_CASE(TLBM_COLL_CUMULANT)
+    TLBM_CASE(TLBM_COLL_GEIER)
+#undef TLBM_CASE
+    default:
+      return fail(TLBM_ERR_INVALID, "unknown collision kind %d", kind);
+  }
+}
+
+template <class L>
+int run_lattice(int store, int kind, bool stream_, bool feat, int lattice, const void* fin, void* fout,
+                int nz, int ny, int nx, const tlbm_collision* coll, const int8_t* solid,
+                const tlbm_boundary* bc, double* force, hipStream_t s) {
+  switch (store) {
+#define TLBM_CASE(S) \
+  case S:            \
+    return run_store<L, S>(kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, s);
+    TLBM_CASE(ST_F32)
+    TLBM_CASE(ST_F64)
+    TLBM_CASE(ST_F16)
+    TLBM_CASE(ST_BF16)
+    TLBM_CASE(ST_F16_DEV)
+    TLBM_CASE(ST_BF16_DEV)
+#undef TLBM_CASE
+    default:
+      return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  }
+}
+
+int run(int lattice, int store, int kind, bool stream_, bool feat, const void* fin, void* fout, int nz,
+        int ny, int nx, const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
+        double* force, hipStream_t s) {
+  switch (lattice) {
+    case TLBM_D2Q9:
+      return run_lattice<D2Q9>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
+                               force, s);
+    case TLBM_D3Q19:
+      return run_lattice<D3Q19>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
+                                force, s);
+    case TLBM_D3Q27:
+      return run_lattice<D3Q27>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
+                                force, s);
+    default:
+      return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", lattice);
+  }
+}
+
+template <class F>
+int by_lattice(int lattice, F&& fn) {
+  switch (lattice) {
+    case TLBM_D2Q9:
+      return fn(D2Q9{});
+    case TLBM_D3Q19:
+      return fn(D3Q19{});
+    case TLBM_D3Q27:
+      return fn(D3Q27{});
+    default:
+      return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", lattice);
+  }
+}
+
+template <class L, class F>
+int by_store(int store, F&& fn) {
+  switch (store) {
+    case ST_F32:
+      return fn(std::integral_constant<int, ST_F32>{});
+    case ST_F64:
+      return fn(std::integral_constant<int, ST_F64>{});
+    case ST_F16:
+      return fn(std::integral_constant<int, ST_F16>{});
+    case ST_BF16:
+      return fn(std::integral_constant<int, ST_BF16>{});
+    case ST_F16_DEV:
+      return fn(std::integral_constant<int, ST_F16_DEV>{});
+    case ST_BF16_DEV:
+      return fn(std::integral_constant<int, ST_BF16_DEV>{});
+    default:
+      return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  }
+}
+
+uint32_t blocks_for(uint64_t n) { return uint32_t((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+int tlbm_version(void) { return 1; }
+
+const char* tlbm_last_error(void) { return g_err; }
+
+const char* tlbm_status_string(int status) {
+  switch (status) {
+    case TLBM_OK:
+      return "ok";
+    case TLBM_ERR_INVALID:
+      return "invalid argument";
+    case TLBM_ERR_SHAPE:
+      return "shape error";
+    case TLBM_ERR_DTYPE:
+      return "unsupported dtype";
+    case TLBM_ERR_UNSUPPORTED:
+      return "unsupported combination";
+    case TLBM_ERR_ALIAS:
+      return "input and output alias";
+    case TLBM_ERR_HIP:
+      return "HIP runtime error";
+    default:
+      return "unknown status";
+  }
+}
+
+int tlbm_step(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
+              tlbm_stream_t stream) {
+  g_err[0] = 0;
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
+  if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
+  if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
+  const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
+  if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
+  return run(lattice, store, coll->kind, true, feat, f_in, f_out, nz, ny, nx, coll, solid, bc, force,
+             (hipStream_t)stream);
+}
+
+int tlbm_stream(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+                tlbm_stream_t stream) {
+  g_err[0] = 0;
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
+  if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; streaming needs two buffers");
+  return run(lattice, store, TLBM_COLL_NONE, true, false, f_in, f_out, nz, ny, nx, nullptr, nullptr, nullptr,
+             nullptr, (hipStream_t)stream);
+}
+
+int tlbm_collide(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
+                 const tlbm_collision* coll, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (store != TLBM_STORE_F32 && store != TLBM_STORE_F64)
+    return fail(TLBM_ERR_DTYPE, "collide-only ops take float32 or float64 populations");
+  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
+  if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return fail(TLBM_ERR_INVALID, "collision kind required");
+  return run(lattice, store, coll->kind, false, false, f_in, f_out, nz, ny, nx, coll, nullptr, nullptr, nullptr,
+             (hipStream_t)stream);
+}
+
+int tlbm_equilibrium(int lattice, int dtype, const void* rho, const void* ux, const void* uy, const void* uz,
+                     void* f_out, int64_t ncells, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
+  if (ncells == 0) return TLBM_OK;
+  if (!rho || !ux || !uy || !f_out) return fail(TLBM_ERR_INVALID, "null pointer");
+  if (lattice != TLBM_D2Q9 && uz == nullptr) return fail(TLBM_ERR_INVALID, "uz required for 3-D lattices");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = uint64_t(ncells);
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    if (dtype == TLBM_STORE_F32) {
+      hipLaunchKernelGGL((k_equilibrium<L, float>), dim3(blocks_for(n)), dim3(256), 0, s, (const float*)rho,
+                         (const float*)ux, (const float*)uy, (const float*)uz, (float*)f_out, n);
+    } else if (dtype == TLBM_STORE_F64) {
+      hipLaunchKernelGGL((k_equilibrium<L, double>), dim3(blocks_for(n)), dim3(256), 0, s, (const double*)rho,
+                         (const double*)ux, (const double*)uy, (const double*)uz, (double*)f_out, n);
+    } else {
+      return fail(TLBM_ERR_DTYPE, "equilibrium takes float32 or float64 fields");
+    }
+    return check_launch("k_equilibrium");
+  });
+}
+
+int tlbm_macroscopic(int lattice, int dtype, const void* f, void* rho, void* ux, void* uy, void* uz,
+                     int64_t ncells, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
+  if (ncells == 0) return TLBM_OK;
+  if (!f || !rho || !ux || !uy) return fail(TLBM_ERR_INVALID, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = uint64_t(ncells);
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    if (dtype == TLBM_STORE_F32) {
+      hipLaunchKernelGGL((k_macroscopic<L, float>), dim3(blocks_for(n)), dim3(256), 0, s, (const float*)f,
+                         (float*)rho, (float*)ux, (float*)uy, (float*)uz, n);
+    } else if (dtype == TLBM_STORE_F64) {
+      hipLaunchKernelGGL((k_macroscopic<L, double>), dim3(blocks_for(n)), dim3(256), 0, s, (const double*)f,
+                         (double*)rho, (double*)ux, (double*)uy, (double*)uz, n);
+    } else {
+      return fail(TLBM_ERR_DTYPE, "macroscopic takes float32 or float64 populations");
+    }
+    return check_launch("k_macroscopic");
+  });
+}
+
+int tlbm_bounce_back(int lattice, int dtype, const void* f_in, void* f_out, const int8_t* mask, int64_t ncells,
+                     tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
+  if (ncells == 0) return TLBM_OK;
+  if (!f_in || !f_out || !mask) return fail(TLBM_ERR_INVALID, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = uint64_t(ncells);
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    if (dtype == TLBM_STORE_F32) {
+      hipLaunchKernelGGL((k_bounce_back<L, float>), dim3(blocks_for(n)), dim3(256), 0, s, (const float*)f_in,
+                         (float*)f_out, mask, n);
+    } else if (dtype == TLBM_STORE_F64) {
+      hipLaunchKernelGGL((k_bounce_back<L, double>), dim3(blocks_for(n)), dim3(256), 0, s, (const double*)f_in,
+                         (double*)f_out, mask, n);
+    } else {
+      return fail(TLBM_ERR_DTYPE, "bounce_back takes float32 or float64 populations");
+    }
+    return check_launch("k_bounce_back");
+  });
+}
+
+static int launch_sums(int lattice, int store, const void* f, const int8_t* solid, int64_t ncells, double* out,
+                       hipStream_t s) {
+  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
+  if (ncells == 0) return TLBM_OK;
+  if (!f || !out) return fail(TLBM_ERR_INVALID, "null pointer");
+  const uint64_t n = uint64_t(ncells);
+  uint32_t nb = blocks_for(n);
+  if (nb > 4096) nb = 4096;
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    return by_store<L>(store, [&](auto sk) -> int {
+      constexpr int S = decltype(sk)::value;
+      hipLaunchKernelGGL((k_moment_sums<L, S>), dim3(nb), dim3(256), 0, s, f, solid, n, out);
+      return check_launch("k_moment_sums");
+    });
+  });
+}
+
+int tlbm_moment_sums(int lattice, int store, const void* f, int64_t ncells, double* out, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  return launch_sums(lattice, store, f, nullptr, ncells, out, (hipStream_t)stream);
+}
+
+int tlbm_obstacle_force(int lattice, int store, const void* f, const int8_t* solid, int64_t ncells, double* out,
+                        tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (!solid) return fail(TLBM_ERR_INVALID, "null solid mask");
+  return launch_sums(lattice, store, f, solid, ncells, out, (hipStream_t)stream);
+}
+
+int tlbm_scale(int lattice, int store, void* f, int64_t ncells, const double* scale, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
+  if (ncells == 0) return TLBM_OK;
+  if (!f || !scale) return fail(TLBM_ERR_INVALID, "null pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = uint64_t(ncells);
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    return by_store<L>(store, [&](auto sk) -> int {
+      constexpr int S = decltype(sk)::value;
+      hipLaunchKernelGGL((k_scale<L, S>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, scale);
+      return check_launch("k_scale");
+    });
+  });
+}
+
+}  // extern "C"

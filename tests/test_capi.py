@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every symbol include/tlbm_hip.h declares (no GPU work)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from tensorlbm_amd import _native as nat
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "tlbm_hip.h"
+
+
+def declared() -> list[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tlbm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared() == nat.exported_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(nat.library_path()))
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_version_and_status_strings():
+    assert nat.lib().tlbm_version() >= 1
+    assert nat.lib().tlbm_status_string(nat.ERR_ALIAS) == b"input and output alias"
+
+
+def test_argument_validation_happens_before_any_launch():
+    """Validation paths return error codes without touching a device."""
+    lib = nat.lib()
+    coll = nat.make_collision(nat.COLL_BGK, 0.7)
+    p = ctypes.c_void_p(0x1000)
+    assert lib.tlbm_step(19, 0, p, p, 4, 4, 4, coll, None, None, None, None) == nat.ERR_ALIAS
+    assert lib.tlbm_step(19, 0, None, p, 4, 4, 4, coll, None, None, None, None) == nat.ERR_INVALID
+    assert lib.tlbm_step(20, 0, p, ctypes.c_void_p(0x2000), 4, 4, 4, coll, None, None, None, None) == nat.ERR_INVALID
+    assert lib.tlbm_step(9, 0, p, ctypes.c_void_p(0x2000), 2, 4, 4, coll, None, None, None, None) == nat.ERR_SHAPE
+    assert lib.tlbm_step(19, 9, p, ctypes.c_void_p(0x2000), 4, 4, 4, coll, None, None, None, None) == nat.ERR_DTYPE
+    bad = nat.make_collision(nat.COLL_BGK, 0.4)
+    assert lib.tlbm_step(19, 0, p, ctypes.c_void_p(0x2000), 4, 4, 4, bad, None, None, None, None) == nat.ERR_INVALID
+    assert lib.tlbm_stream(19, 0, p, p, 4, 4, 4, None) == nat.ERR_ALIAS
+    assert lib.tlbm_collide(19, 2, p, p, 4, 4, 4, coll, None) == nat.ERR_DTYPE
+    big = 1 << 11
+    assert lib.tlbm_step(19, 0, p, ctypes.c_void_p(0x2000), big, big, big, coll, None, None, None, None) == nat.ERR_SHAPE
+
+
+def test_check_maps_status_to_reference_exceptions():
+    with pytest.raises(ValueError):
+        nat.check(nat.ERR_SHAPE, "x")
+    with pytest.raises(NotImplementedError):
+        nat.check(nat.ERR_UNSUPPORTED, "x")
+    with pytest.raises(RuntimeError):
+        nat.check(nat.ERR_HIP, "x")
+
+
+def test_store_codes():
+    import torch
+
+    assert nat.store_code(torch.float16, deviation=True) == nat.STORE_F16_DEV
+    assert nat.store_code(torch.bfloat16) == nat.STORE_BF16
+    with pytest.raises(ValueError, match="Unsupported dtype"):
+        nat.store_code(torch.int32)
+    with pytest.raises(ValueError):
+        nat.store_code(torch.float32, deviation=True)

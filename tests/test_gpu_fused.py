@@ -1,0 +1,193 @@
+"""GPU parity of the fused step, stream and collide ops against the oracle.
+
+Mirrors the reference's tests/test_triton_fused.py (stream bit-exact vs roll,
+fused vs separate, mass drift, alias race, reduced precision) and
+tests/test_triton_fused_asymmetric.py (12-step trajectories on
+z-asymmetric fields at rtol 1e-4 / atol 1e-6).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import tensorlbm_amd as tl
+from oracle import ref
+from tests import _fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(a: np.ndarray, dev, dtype=torch.float32) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dtype)
+
+
+def _np(t: torch.Tensor) -> np.ndarray:
+    return t.detach().cpu().to(torch.float64).numpy() if t.dtype in (torch.float16, torch.bfloat16) else t.detach().cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# streaming: a permutation, so bit-exact
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("q,shape", [(9, (1, 24, 40)), (19, (8, 12, 20)), (27, (6, 10, 16)), (19, (1, 1, 5)),
+                                     (19, (3, 2, 130))])
+def test_stream_integer_canary_bit_exact(q, shape, dev):
+    f = F.canary(q, *shape)
+    if q == 9:
+        f = f[:, 0]
+    got = _np(tl.hip_stream(_gpu(f, dev)))
+    want = ref.stream(f.astype(np.float32))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.float16, torch.bfloat16])
+def test_stream_bit_exact_vs_roll_all_storages(dtype, dev):
+    f = _gpu(F.random_state(19, (9, 11, 17), seed=3), dev, dtype)
+    got = tl.stream3d(f)
+    ref_t = torch.empty_like(f)
+    c = tl.lattice.D3Q19.C
+    for q in range(19):
+        ref_t[q] = torch.roll(f[q], shifts=(int(c[q, 2]), int(c[q, 1]), int(c[q, 0])), dims=(0, 1, 2))
+    assert torch.equal(got, ref_t)
+
+
+# ---------------------------------------------------------------------------
+# fused BGK trajectories (tests/test_triton_fused_asymmetric.py:178-207)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("field", sorted(F.ASYM_FIELDS))
+def test_fused_matches_oracle_on_asymmetric_fields(field, dev):
+    n, tau, steps = 32, 0.7, 12
+    f0 = F.ASYM_FIELDS[field](n).astype(np.float32)
+    solver = tl.FusedSolver3D(n, n, n, tau=tau, device=dev)
+    fg = _gpu(f0, dev)
+    fr = f0
+    for _ in range(steps):
+        fg = solver.step(fg)
+        fr = ref.fused_step(fr, tau)
+    got = _np(fg)
+    assert np.allclose(got, fr, rtol=1e-4, atol=1e-6), float(np.abs(got - fr).max())
+
+
+@pytest.mark.parametrize("q,shape", [(9, (1, 64, 48)), (27, (12, 10, 14)), (19, (7, 9, 130))])
+def test_fused_other_lattices_and_ragged_shapes(q, shape, dev):
+    f0 = F.random_state(q, shape, seed=11).astype(np.float32)
+    if q == 9:
+        f0 = f0[:, 0]
+    fg, fr = _gpu(f0, dev), f0
+    for _ in range(5):
+        fg = tl.fused_step(fg, 0.62)
+        fr = ref.fused_step(fr, 0.62)
+    assert np.allclose(_np(fg), fr, rtol=1e-4, atol=1e-6), float(np.abs(_np(fg) - fr).max())
+
+
+def test_fused_matches_separate_collide_stream(dev):
+    f = _gpu(F.random_state(19, (16, 16, 16), seed=7, noise=5e-3), dev)
+    sep = tl.hip_collide(tl.hip_stream(f), 0.55)
+    fused = tl.hip_fused(f, 0.55)
+    rel = float((sep - fused).abs().max() / sep.abs().max())
+    assert rel < 1e-4
+
+
+def test_fp64_run_matches_fp64_oracle(dev):
+    f0 = F.random_state(19, (10, 12, 14), seed=5)
+    fg, fr = _gpu(f0, dev, torch.float64), f0
+    for _ in range(4):
+        fg = tl.fused_step(fg, 0.6)
+        fr = ref.fused_step(fr, 0.6)
+    assert np.allclose(_np(fg), fr, rtol=1e-13, atol=1e-15)
+
+
+def test_mass_conservation_100_steps(dev):
+    n = 32
+    solver = tl.FusedSolver3D(n, n, n, tau=0.6, device=dev)
+    f = _gpu(F.random_state(19, (n, n, n), seed=0), dev)
+    m0 = float(f.double().sum())
+    for _ in range(100):
+        f = solver.step(f)
+    assert abs(float(f.double().sum()) - m0) / m0 < 1e-4
+
+
+def test_ping_pong_no_alias_race_on_blob(dev):
+    """tests/test_triton_fused.py:174-233: solver.step output fed back == fresh-buffer steps."""
+    n, tau, steps = 64, 0.8, 50
+    a = np.arange(n) - n // 2
+    z, y, x = np.meshgrid(a, a, a, indexing="ij")
+    rho = 1.0 + 0.5 * np.exp(-(z * z + y * y + x * x) / 32.0)
+    f0 = ref.equilibrium(ref.D3Q19, rho, np.full_like(rho, 0.03), np.full_like(rho, 0.02),
+                         np.full_like(rho, 0.01)).astype(np.float32)
+    solver = tl.FusedSolver3D(n, n, n, tau=tau, device=dev)
+    x1 = solver.step(_gpu(f0, dev))
+    x2 = solver.step(x1)
+    assert x1.data_ptr() != x2.data_ptr()
+    f = _gpu(f0, dev)
+    g = _gpu(f0, dev)
+    for _ in range(steps):
+        f = solver.step(f)
+        g = tl.hip_fused(g, tau)
+    assert torch.equal(f, g)
+
+
+def test_alias_rejected(dev):
+    f = _gpu(F.random_state(19, (4, 4, 4)), dev)
+    with pytest.raises(ValueError, match="alias"):
+        tl.hip_fused(f, 0.7, out=f)
+
+
+def test_solver_rejects_bad_dtype_and_cpu():
+    with pytest.raises(RuntimeError):
+        tl.FusedSolver3D(8, 8, 8, tau=0.6, device="cpu")
+
+
+# ---------------------------------------------------------------------------
+# reduced-precision storage (tests/test_triton_fused.py:241-256)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dtype,bound", [(torch.float16, 1e-6), (torch.bfloat16, 1e-4)])
+def test_reduced_precision_mass_conserved(dtype, bound, dev):
+    n = 32
+    f = _gpu(F.random_state(19, (n, n, n), seed=0), dev, dtype)
+    solver = tl.FusedSolver3D(n, n, n, tau=0.6, device=dev, dtype=dtype)
+    m0 = float(f.double().sum())
+    f = solver.step(f)
+    assert abs(float(f.double().sum()) - m0) / m0 < bound
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+@pytest.mark.parametrize("deviation", [False, True])
+def test_16bit_storage_error_vs_fp64(dtype, deviation, dev):
+    """20 steps of 16-bit storage vs the fp64 oracle; deviation storage must beat plain storage."""
+    n, tau, steps = 24, 0.6, 20
+    f0 = F.random_state(19, (n, n, n), seed=2, noise=2e-3)
+    wv = ref.D3Q19.w.reshape(-1, 1, 1, 1)
+    init = (f0 - wv) if deviation else f0
+    fg = _gpu(init, dev, dtype)
+    solver = tl.FusedSolver3D(n, n, n, tau=tau, device=dev, dtype=dtype, deviation=deviation)
+    fr = f0
+    for _ in range(steps):
+        fg = solver.step(fg)
+        fr = ref.fused_step(fr, tau)
+    got = _np(fg) + (wv if deviation else 0.0)
+    _, ux, uy, uz = ref.macroscopic(got)
+    _, rx, ry, rz = ref.macroscopic(fr)
+    err = np.sqrt(((ux - rx) ** 2 + (uy - ry) ** 2 + (uz - rz) ** 2).mean() / (rx**2 + ry**2 + rz**2).mean())
+    print(f"{dtype} deviation={deviation}: velocity rel-L2 error vs fp64 after {steps} steps = {err:.3e}")
+    limit = {(torch.float16, True): 1e-3, (torch.bfloat16, True): 1e-2,
+             (torch.float16, False): 1e-1, (torch.bfloat16, False): 1.0}[(dtype, deviation)]
+    assert err < limit, err
+
+
+def test_moment_sums_and_correct_mass(dev):
+    f0 = F.random_state(19, (6, 7, 8), seed=9)
+    fg = _gpu(f0, dev, torch.float64)
+    s = tl.moment_sums(fg).cpu().numpy()
+    rho, ux, uy, uz = ref.macroscopic(f0)
+    assert np.allclose(s, [rho.sum(), (rho * ux).sum(), (rho * uy).sum(), (rho * uz).sum()], rtol=1e-12)
+    out = tl.correct_mass3d(fg, 100.0)
+    assert abs(float(out.sum()) - 100.0) < 1e-9
+    assert np.allclose(_np(out), ref.correct_mass(f0, 100.0), rtol=1e-12)
