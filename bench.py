@@ -119,6 +119,19 @@ def cpu_baseline(n: int, tau: float, seconds: float):
     }
 
 
+PMC_SUMMARY = ROOT / "profiles" / "k_lbm_d3q19_bgk_fp32_channel256_pmc.md"
+
+
+def measured_traffic(n: int):
+    """HBM bytes per launch of the headline kernel from the committed PMC summary (256^3 only)."""
+    if n != 256 or not PMC_SUMMARY.is_file():
+        return None
+    for line in PMC_SUMMARY.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -206,7 +219,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; profiles/k_lbm_d3q19_bgk_fp32_channel256_pmc.md",
             "bytes_per_cell": head["bytes_per_cell"],
             "kernel": "k_lbm<D3Q19, fp32, BGK, stream, features>",
         },

@@ -94,51 +94,63 @@ const char* tlbm_last_error(void);
 const char* tlbm_status_string(int status);
 
 /*
+ * Population layouts: ``strides`` point to 3 int64 element strides (q, z, y);
+ * the x stride is 1.  NULL means the reference's contiguous f[Q][nz][ny][nx].
+ * One z-plane (y*sy + x) must span less than 4 GiB.  Input and output of a
+ * streaming op may use different layouts (e.g. the solver's padded
+ * [z][q][y][x] buffers); they must not overlap.
+ */
+
+/*
  * Fused pull-stream + boundary + collide, one launch
  *   (replaces triton_fused()          src/tensorlbm/triton_fused.py:618-667,
  *             triton_fused_obstacle_xfar_les()  triton_fused_obstacle.py:1695-2053).
- * solid  : optional int8 (nz,ny,nx) mask, non-zero = solid; full-way bounce-back.
+ * solid  : optional int8 (nz,ny,nx) contiguous mask, non-zero = solid; full-way bounce-back.
  * bc     : optional boundary (NULL = periodic).
  * force  : optional 3 doubles on the device; the Ladd force of the post-stream,
  *          pre-bounce-back populations on solid cells is ADDED (caller zeroes).
  */
-int tlbm_step(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
-              double* force, tlbm_stream_t stream);
+int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+              const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,
+              const int8_t* solid, const tlbm_boundary* bc, double* force, tlbm_stream_t stream);
 
 /* periodic pull stream (solver3d.py:210-239 stream3d, solver.py:396-417, d3q27.py:766-791) */
-int tlbm_stream(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-                tlbm_stream_t stream);
+int tlbm_stream(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+                const int64_t* out_strides, int nz, int ny, int nx, tlbm_stream_t stream);
 
-/* local collision, f_out may equal f_in (collide_bgk3d, collide_mrt3d, ...) */
-int tlbm_collide(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-                 const tlbm_collision* coll, tlbm_stream_t stream);
+/* local collision, f_out may equal f_in (collide_bgk3d, collide_mrt3d, ...); F32 / F64 */
+int tlbm_collide(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+                 const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,
+                 tlbm_stream_t stream);
 
 /* equilibrium (d2q9.py:39-58, d3q19.py:83-125, d3q27.py:353-392); dtype F32 or F64;
- * uz may be NULL (D2Q9). */
+ * contiguous fields; uz may be NULL (D2Q9). */
 int tlbm_equilibrium(int lattice, int dtype, const void* rho, const void* ux, const void* uy,
                      const void* uz, void* f_out, int64_t ncells, tlbm_stream_t stream);
 
-/* macroscopic moments (d2q9.py:61-74, d3q19.py:165-186, d3q27.py:395-420); uz may be NULL. */
+/* macroscopic moments (d2q9.py:61-74, d3q19.py:165-186, d3q27.py:395-420); contiguous;
+ * uz may be NULL. */
 int tlbm_macroscopic(int lattice, int dtype, const void* f, void* rho, void* ux, void* uy, void* uz,
                      int64_t ncells, tlbm_stream_t stream);
 
-/* full-way swap where(mask, f[opp], f) (boundaries3d.py:112-132); f_out may equal f_in */
+/* full-way swap where(mask, f[opp], f) (boundaries3d.py:112-132); contiguous;
+ * f_out may equal f_in */
 int tlbm_bounce_back(int lattice, int dtype, const void* f_in, void* f_out, const int8_t* mask,
                      int64_t ncells, tlbm_stream_t stream);
 
 /* ADD to out[0..3] (device doubles): sum f, sum c_x f, sum c_y f, sum c_z f (FP64 wave
- * reductions).  Works on every storage (deviation storages add back w). */
-int tlbm_moment_sums(int lattice, int store, const void* f, int64_t ncells, double* out,
-                     tlbm_stream_t stream);
+ * reductions, correct_mass3d's sum, solver3d.py:287-300).  Any storage and layout;
+ * deviation storages add back w. */
+int tlbm_moment_sums(int lattice, int store, const void* f, const int64_t* strides, int nz, int ny,
+                     int nx, double* out, tlbm_stream_t stream);
 
 /* ADD the Ladd force 2 sum_solid sum_q c_q f to out[0..3] (obstacles.py:91-130), FP64. */
-int tlbm_obstacle_force(int lattice, int store, const void* f, const int8_t* solid,
-                        int64_t ncells, double* out, tlbm_stream_t stream);
+int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* strides, int nz,
+                        int ny, int nx, const int8_t* solid, double* out, tlbm_stream_t stream);
 
 /* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
-int tlbm_scale(int lattice, int store, void* f, int64_t ncells, const double* scale,
-               tlbm_stream_t stream);
+int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx,
+               const double* scale, tlbm_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -63,16 +63,16 @@ _SIGNATURES = {
     "tlbm_version": ([], _I),
     "tlbm_last_error": ([], ctypes.c_char_p),
     "tlbm_status_string": ([_I], ctypes.c_char_p),
-    "tlbm_step": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP,
+    "tlbm_step": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP,
                    ctypes.POINTER(Boundary), _VP, _VP], _I),
-    "tlbm_stream": ([_I, _I, _VP, _VP, _I, _I, _I, _VP], _I),
-    "tlbm_collide": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP], _I),
+    "tlbm_stream": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, _VP], _I),
+    "tlbm_collide": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP], _I),
     "tlbm_equilibrium": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I64, _VP], _I),
     "tlbm_macroscopic": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I64, _VP], _I),
     "tlbm_bounce_back": ([_I, _I, _VP, _VP, _VP, _I64, _VP], _I),
-    "tlbm_moment_sums": ([_I, _I, _VP, _I64, _VP, _VP], _I),
-    "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I64, _VP, _VP], _I),
-    "tlbm_scale": ([_I, _I, _VP, _I64, _VP, _VP], _I),
+    "tlbm_moment_sums": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP, _VP], _I),
+    "tlbm_scale": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
 }
 
 _lib: ctypes.CDLL | None = None
@@ -122,6 +122,28 @@ def check(rc: int, what: str) -> None:
 
 def ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+Strides = ctypes.c_int64 * 3
+
+
+def grid_of(f: torch.Tensor) -> tuple[int, int, int, int]:
+    """(Q, nz, ny, nx) of a population tensor (D2Q9 fields are (9, ny, nx))."""
+    q = int(f.shape[0])
+    if q == 9 and f.ndim == 3:
+        return q, 1, int(f.shape[1]), int(f.shape[2])
+    if q in (19, 27) and f.ndim == 4:
+        return q, int(f.shape[1]), int(f.shape[2]), int(f.shape[3])
+    raise ValueError(f"populations must be (9, ny, nx), (19, nz, ny, nx) or (27, nz, ny, nx); got {tuple(f.shape)}")
+
+
+def strides_of(f: torch.Tensor) -> "Strides":
+    """(q, z, y) element strides of a population tensor whose x stride is 1."""
+    if f.stride(-1) != 1:
+        raise ValueError("populations need a unit stride along x")
+    if f.ndim == 3:
+        return Strides(f.stride(0), 0, f.stride(1))
+    return Strides(f.stride(0), f.stride(1), f.stride(2))
 
 
 def stream_of(t: torch.Tensor) -> int:

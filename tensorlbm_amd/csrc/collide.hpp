@@ -15,6 +15,10 @@ struct Macro {
   C rho, rho_safe, ux, uy, uz;
 };
 
+// 1/x: one v_rcp_f32 (1 ulp) in float, exact division in double
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // rho = sum f, u = sum c f / max(rho, 1e-12)   (d3q19.py:165-186)
 template <class L, typename C>
 __device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
@@ -29,7 +33,7 @@ __device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
   Macro<C> m;
   m.rho = rho;
   m.rho_safe = rho > C(1e-12) ? rho : C(1e-12);
-  const C inv = C(1) / m.rho_safe;
+  const C inv = recip(m.rho_safe);
   m.ux = jx * inv;
   m.uy = jy * inv;
   m.uz = jz * inv;
@@ -48,6 +52,49 @@ __device__ __forceinline__ void equilibrium(C rho, C ux, C uy, C uz, C (&feq)[L:
     if (L::cz[q] != 0) cu += C(L::cz[q]) * uz;
     feq[q] = C(L::w[q]) * rho * (C(1) + C(3) * cu + C(4.5) * cu * cu - usq15);
   }
+}
+
+// BGK relaxation f <- (1-w) f + w f_eq with the equilibrium evaluated per
+// opposite pair: f_eq(q), f_eq(opp q) = s +/- a with s = w_q rho (1 - 1.5u^2 + 4.5 cu^2)
+// and a = 3 w_q rho cu (same algebra as d3q19.py:119-121, fewer operations).
+template <class L, typename C>
+__device__ __forceinline__ void bgk_relax(C (&f)[L::Q], C rho, C ux, C uy, C uz, C omega) {
+  const C om1 = C(1) - omega;
+  const C orho = omega * rho;
+  const C base = orho * (C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz));
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int o = L::opp[q];
+    if (o == q) {
+      f[q] = om1 * f[q] + C(L::w[q]) * base;
+    } else if (q < o) {
+      C cu = C(0);
+      if (L::cx[q] != 0) cu += C(L::cx[q]) * ux;
+      if (L::cy[q] != 0) cu += C(L::cy[q]) * uy;
+      if (L::cz[q] != 0) cu += C(L::cz[q]) * uz;
+      const C k = C(L::w[q]) * orho;
+      const C sym = (C(4.5) * k) * cu * cu + C(L::w[q]) * base;
+      const C asym = (C(3) * k) * cu;
+      f[q] = om1 * f[q] + (sym + asym);
+      f[o] = om1 * f[o] + (sym - asym);
+    }
+  }
+}
+
+// Zou-He moving lid on the y = ny-1 plane, ux = u, uy = uz = 0 (boundaries3d.py:324-404)
+template <typename C>
+__device__ __forceinline__ void zou_he_lid(C (&f)[19], C u) {
+  const C s0 = f[0] + f[1] + f[2] + f[5] + f[6] + f[11] + f[12] + f[13] + f[14];
+  const C sp = f[3] + f[7] + f[10] + f[15] + f[17];
+  const C rho = s0 + C(2) * sp;
+  const C dx = rho * u - ((f[1] - f[2] + f[11] - f[12] + f[13] - f[14]) + (f[7] - f[10]));
+  const C sz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14];
+  const C h = C(0.5) * (f[7] + f[10]);
+  f[4] = f[3];
+  f[8] = h - C(0.5) * dx;
+  f[9] = h + C(0.5) * dx;
+  f[16] = f[15] + C(0.5) * sz;
+  f[18] = f[17] - C(0.5) * sz;
 }
 
 // Second-order non-equilibrium stress Pi_ab = sum c_a c_b fneq (xx yy zz xy xz yz)

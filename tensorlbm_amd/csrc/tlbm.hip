@@ -2,12 +2,25 @@
 //
 // One kernel template (k_lbm) covers the fused pull-stream + boundary +
 // bounce-back + collide step, the stream-only op and the collide-only op for
-// D2Q9 / D3Q19 / D3Q27 and all storage formats.  Each thread owns one cell:
-// it gathers the Q incoming populations (coalesced along x; a +-1 x shift
-// costs one extra cache line per row, served by L2), collides in registers
-// and writes Q coalesced values at its own cell.  Full-way bounce-back at a
-// solid cell is a register lane swap f[q] <-> f[opp(q)] of the pulled values
-// (pull[opp q](x) = b[opp q][x + c_q]), so walls cost no extra loads.
+// D2Q9 / D3Q19 / D3Q27 and all storage formats.
+//
+// Work mapping: a workgroup of 256 lanes covers a (BX x BY) tile of one
+// z-plane, x fastest, one cell per lane.  Each lane gathers its Q incoming
+// populations (coalesced along x; the +-1 x shift costs one partial cache line
+// per row, absorbed by L2 so HBM traffic stays at 2 Q itemsize per cell),
+// collides in registers and streams Q values out with non-temporal stores.
+//
+// Addressing: populations may use any (q, z, y) strides with unit x stride.
+// For lane q the plane base  f + q*sq + z_src*sz  is uniform over the
+// workgroup (64-bit, scalar registers) and only  y_src*sy + x_src  is a
+// per-lane 32-bit offset, so every load/store uses the scalar-base +
+// 32-bit-offset form.  The solver keeps its buffers in a [z][q][y][x] layout
+// with a padded q stride (fewer concurrently open DRAM rows than plain SoA);
+// the reference's contiguous [q][z][y][x] tensors are served by the same code.
+//
+// Full-way bounce-back at a solid cell is a register lane swap
+// f[q] <-> f[opp(q)] of the pulled values (pull[opp q](x) = b[opp q][x + c_q]),
+// so walls cost no extra loads.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdint.h>
@@ -32,7 +45,8 @@ struct KArgs {
   const float* tau_eff;
   const void* mlp;
   double* force;
-  uint64_t N;
+  int64_t isq, isz, isy;  // input strides (elements; x stride 1)
+  int64_t osq, osz, osy;  // output strides
   int nx, ny, nz, tiles_x, tiles_y;
   int tau_mode, eq_floor, bc_kind;
   C omega, tau, cs_delta_sq, tau_cap;
@@ -41,30 +55,39 @@ struct KArgs {
   C bc_u[3];
 };
 
-template <class ST, class L>
-__device__ __forceinline__ typename ST::C ld(const typename ST::T* p, int q, uint64_t N, uint32_t i) {
-  return ST::template get<L>(p[uint64_t(q) * N + i], q);
+// value at a uniform base plus a per-lane 32-bit element offset
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off * uint32_t(sizeof(T)));
 }
 
-template <class ST, class L>
-__device__ __forceinline__ void st(typename ST::T* p, int q, uint64_t N, uint32_t i, typename ST::C v) {
-  p[uint64_t(q) * N + i] = ST::template put<L>(v, q);
+template <typename T>
+__device__ __forceinline__ void st_nt(T* base, uint32_t off, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off * uint32_t(sizeof(T))));
 }
 
-// source cell of lane q for a pull at (x, y, z) with periodic wrap
-template <class L>
-__device__ __forceinline__ uint32_t pull_src(int q, int x, int y, int z, int nx, int ny, int nz) {
-  int sx = x - L::cx[q], sy = y - L::cy[q], sz = z - L::cz[q];
-  sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
-  sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
-  sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
-  return (uint32_t(sz) * uint32_t(ny) + uint32_t(sy)) * uint32_t(nx) + uint32_t(sx);
-}
+__device__ __forceinline__ int wrap_dn(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+__device__ __forceinline__ int wrap_up(int v, int n) { return v == n - 1 ? 0 : v + 1; }
+
+// source coordinate of lane velocity component c for a pull: x - c with wrap
+__device__ __forceinline__ int src_of(int c, int dn, int mid, int up) { return c > 0 ? dn : (c < 0 ? up : mid); }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// pulled population q of cell (x, y, z) (periodic wrap), used off the hot loop
+template <class ST, class L, typename C>
+__device__ __forceinline__ typename ST::C pull_one(const typename ST::T* fin, const KArgs<C>& a, int q, int x, int y,
+                                                   int z) {
+  int sx = x - L::cx[q], sy = y - L::cy[q], sz = z - L::cz[q];
+  sx = sx < 0 ? sx + a.nx : (sx >= a.nx ? sx - a.nx : sx);
+  sy = sy < 0 ? sy + a.ny : (sy >= a.ny ? sy - a.ny : sy);
+  sz = sz < 0 ? sz + a.nz : (sz >= a.nz ? sz - a.nz : sz);
+  const typename ST::T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
+  return ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
 }
 
 template <class L, int S, int COLL, bool STREAM, int FEAT>
@@ -82,23 +105,31 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
   const int x = int(txi * blockDim.x + threadIdx.x);
   const int y = int(tyi * blockDim.y + threadIdx.y);
   const bool active = x < a.nx && y < a.ny;
-  constexpr bool kForcePath = (FEAT == FEAT_ALL);
-  if (!kForcePath && !active) return;
+  if (FEAT != FEAT_ALL && !active) return;
   // inactive lanes (force path only) work on cell (0, 0, z): in bounds, never stored
   const int xx = active ? x : 0;
   const int yy = active ? y : 0;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const uint64_t N = a.N;
   const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(yy)) * uint32_t(nx) + uint32_t(xx);
   const T* fin = static_cast<const T*>(a.fin);
 
   C f[Q];
   if constexpr (STREAM) {
+    const int xd = wrap_dn(xx, nx), xu = wrap_up(xx, nx);
+    const int yd = wrap_dn(yy, ny), yu = wrap_up(yy, ny);
+    const int zd = wrap_dn(z, nz), zu = wrap_up(z, nz);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, pull_src<L>(q, xx, yy, z, nx, ny, nz));
+    for (int q = 0; q < Q; ++q) {
+      const int sx = src_of(L::cx[q], xd, xx, xu);
+      const int sy = src_of(L::cy[q], yd, yy, yu);
+      const int sz = src_of(L::cz[q], zd, z, zu);
+      const T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
+      f[q] = ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
+    }
   } else {
+    const uint32_t off = uint32_t(yy) * uint32_t(a.isy) + uint32_t(xx);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, cell);
+    for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(ld_off(fin + int64_t(q) * a.isq + int64_t(z) * a.isz, off), q);
   }
 
   bool solid = false;
@@ -119,28 +150,15 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
         // equilibrium inlet, density of the pulled state at x = 1
         C rho1 = C(0);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) rho1 += ld<ST, L>(fin, q, N, pull_src<L>(q, 1, yy, z, nx, ny, nz));
+        for (int q = 0; q < Q; ++q) rho1 += pull_one<ST, L>(fin, a, q, 1, yy, z);
         equilibrium<L>(rho1, a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
       } else if (xx == nx - 1) {
         // zero-gradient outlet: copy the pulled state of x = nx-2
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = ld<ST, L>(fin, q, N, pull_src<L>(q, nx - 2, yy, z, nx, ny, nz));
+        for (int q = 0; q < Q; ++q) f[q] = pull_one<ST, L>(fin, a, q, nx - 2, yy, z);
       }
     } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1) {
-      // Zou-He moving lid, ux = u_lid (boundaries3d.py:324-404)
-      if constexpr (Q == 19) {
-      const C s0 = f[0] + f[1] + f[2] + f[5] + f[6] + f[11] + f[12] + f[13] + f[14];
-      const C sp = f[3] + f[7] + f[10] + f[15] + f[17];
-      const C rho = s0 + C(2) * sp;
-      const C dx = rho * a.bc_u[0] - ((f[1] - f[2] + f[11] - f[12] + f[13] - f[14]) + (f[7] - f[10]));
-      const C sz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14];
-      const C h = C(0.5) * (f[7] + f[10]);
-      f[4] = f[3];
-      f[8] = h - C(0.5) * dx;
-      f[9] = h + C(0.5) * dx;
-      f[16] = f[15] + C(0.5) * sz;
-      f[18] = f[17] - C(0.5) * sz;
-      }
+      if constexpr (Q == 19) zou_he_lid(f, a.bc_u[0]);
     }
     if (solid) {
 #pragma unroll
@@ -156,44 +174,58 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
 
   if constexpr (COLL != TLBM_COLL_NONE) {
     const Macro<C> mc = macro<L>(f);
-    C feq[Q];
-    equilibrium<L>(a.eq_floor ? mc.rho_safe : mc.rho, mc.ux, mc.uy, mc.uz, feq);
+    const C rho_eq = a.eq_floor ? mc.rho_safe : mc.rho;
     C omega = a.omega;
-    C pi[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
     const bool need_pi = (COLL == TLBM_COLL_CUMULANT) || a.tau_mode == TLBM_TAU_SMAG ||
                          a.tau_mode == TLBM_TAU_SMAG_RAW;
-    if (need_pi) {
-      C fneq[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
-      neq_stress<L>(fneq, pi);
-    }
-    if (a.tau_mode == TLBM_TAU_SMAG) {
-      omega = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
-    } else if (a.tau_mode == TLBM_TAU_SMAG_RAW) {
-      omega = smagorinsky_raw_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq);
-    } else if (a.tau_mode == TLBM_TAU_FIELD) {
-      omega = C(1) / C(a.tau_eff[cell]);
-    }
-
     if constexpr (COLL == TLBM_COLL_BGK) {
+      if (need_pi) {
+        C feq[Q], fneq[Q], pi[6];
+        equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) f[q] = f[q] - omega * (f[q] - feq[q]);
-    } else if constexpr (COLL == TLBM_COLL_MRT) {
-      mrt<L>(f, feq, a.s, omega);
-    } else if constexpr (COLL == TLBM_COLL_CM) {
-      if constexpr (Q != 9) cascaded<L>(f, feq, mc, omega, a.rate);
-    } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
-      cumulant19<L>(f, feq, pi, omega, a.rate);
-    } else if constexpr (COLL == TLBM_COLL_GEIER) {
-      if constexpr (Q == 27) geier27<C>(f, feq, mc, omega, a.rate);
+        for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
+        neq_stress<L>(fneq, pi);
+        omega = a.tau_mode == TLBM_TAU_SMAG
+                    ? smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap)
+                    : smagorinsky_raw_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq);
+      } else if (a.tau_mode == TLBM_TAU_FIELD) {
+        omega = C(1) / C(a.tau_eff[cell]);
+      }
+      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, omega);
+    } else {
+      C feq[Q];
+      equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
+      C pi[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+      if (need_pi) {
+        C fneq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
+        neq_stress<L>(fneq, pi);
+      }
+      if (a.tau_mode == TLBM_TAU_SMAG) {
+        omega = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
+      } else if (a.tau_mode == TLBM_TAU_SMAG_RAW) {
+        omega = smagorinsky_raw_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq);
+      } else if (a.tau_mode == TLBM_TAU_FIELD) {
+        omega = C(1) / C(a.tau_eff[cell]);
+      }
+      if constexpr (COLL == TLBM_COLL_MRT) {
+        mrt<L>(f, feq, a.s, omega);
+      } else if constexpr (COLL == TLBM_COLL_CM) {
+        if constexpr (Q != 9) cascaded<L>(f, feq, mc, omega, a.rate);
+      } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
+        cumulant19<L>(f, feq, pi, omega, a.rate);
+      } else if constexpr (COLL == TLBM_COLL_GEIER) {
+        if constexpr (Q == 27) geier27<C>(f, feq, mc, omega, a.rate);
+      }
     }
   }
 
   if (active) {
     T* fout = static_cast<T*>(a.fout);
+    const uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) st<ST, L>(fout, q, N, cell, f[q]);
+    for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
   }
 
   if constexpr (FEAT == FEAT_ALL) {
@@ -213,7 +245,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
 }
 
 // ---------------------------------------------------------------------------
-// elementwise ops
+// elementwise ops (contiguous fields)
 // ---------------------------------------------------------------------------
 
 template <class L, typename T>
@@ -253,19 +285,34 @@ __global__ __launch_bounds__(256) void k_bounce_back(const T* fin, T* fout, cons
   for (int q = 0; q < L::Q; ++q) fout[uint64_t(q) * N + i] = s ? v[L::opp[q]] : v[q];
 }
 
-// sums of f, c_x f, c_y f, c_z f (solid != nullptr: Ladd force over solid cells, x2)
+// ---------------------------------------------------------------------------
+// layout-aware FP64 reductions and scaling
+// ---------------------------------------------------------------------------
+
+struct Layout {
+  int64_t sq, sz, sy;
+  int nx, ny, nz;
+};
+
+// sums of f, c_x f, c_y f, c_z f over all cells (solid != nullptr: only solid
+// cells, Ladd factor 2, out[0..2] = force)
 template <class L, int S>
-__global__ __launch_bounds__(256) void k_moment_sums(const void* fv, const int8_t* solid, uint64_t N,
-                                                     double* out) {
+__global__ __launch_bounds__(256) void k_moment_sums(const void* fv, Layout lay, const int8_t* solid, double* out) {
   using ST = Store<S>;
   const typename ST::T* f = static_cast<const typename ST::T*>(fv);
   double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  const uint64_t N = uint64_t(lay.nx) * uint64_t(lay.ny) * uint64_t(lay.nz);
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
     if (solid != nullptr && solid[i] == 0) continue;
+    const uint64_t x = i % uint64_t(lay.nx);
+    const uint64_t r = i / uint64_t(lay.nx);
+    const uint64_t y = r % uint64_t(lay.ny);
+    const uint64_t z = r / uint64_t(lay.ny);
+    const int64_t base = int64_t(z) * lay.sz + int64_t(y) * lay.sy + int64_t(x);
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
-      const double v = double(ST::template get<L>(f[uint64_t(q) * N + i], q));
+      const double v = double(ST::template get<L>(f[base + int64_t(q) * lay.sq], q));
       s0 += v;
       if (L::cx[q] != 0) sx += double(L::cx[q]) * v;
       if (L::cy[q] != 0) sy += double(L::cy[q]) * v;
@@ -288,22 +335,30 @@ __global__ __launch_bounds__(256) void k_moment_sums(const void* fv, const int8_
   if (threadIdx.x < 4) {
     double t = 0.0;
     for (int w = 0; w < int(blockDim.x / 64); ++w) t += part[w][threadIdx.x];
-    const double k = (solid != nullptr && threadIdx.x > 0) ? 2.0 : 1.0;
-    if (solid != nullptr && threadIdx.x == 0) return;
-    atomicAdd(out + (solid != nullptr ? threadIdx.x - 1 : threadIdx.x), k * t);
+    if (solid == nullptr) {
+      atomicAdd(out + threadIdx.x, t);
+    } else if (threadIdx.x > 0) {
+      atomicAdd(out + threadIdx.x - 1, 2.0 * t);
+    }
   }
 }
 
 template <class L, int S>
-__global__ __launch_bounds__(256) void k_scale(void* fv, uint64_t N, const double* scale) {
+__global__ __launch_bounds__(256) void k_scale(void* fv, Layout lay, const double* scale) {
   using ST = Store<S>;
   typename ST::T* f = static_cast<typename ST::T*>(fv);
+  const uint64_t N = uint64_t(lay.nx) * uint64_t(lay.ny) * uint64_t(lay.nz);
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const uint64_t x = i % uint64_t(lay.nx);
+  const uint64_t r = i / uint64_t(lay.nx);
+  const uint64_t y = r % uint64_t(lay.ny);
+  const uint64_t z = r / uint64_t(lay.ny);
+  const int64_t base = int64_t(z) * lay.sz + int64_t(y) * lay.sy + int64_t(x);
   const typename ST::C s = typename ST::C(*scale);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
-    const uint64_t k = uint64_t(q) * N + i;
+    const int64_t k = base + int64_t(q) * lay.sq;
     f[k] = ST::template put<L>(ST::template get<L>(f[k], q) * s, q);
   }
 }
@@ -338,6 +393,17 @@ int check_launch(const char* what) {
 bool lattice_ok(int l) { return l == TLBM_D2Q9 || l == TLBM_D3Q19 || l == TLBM_D3Q27; }
 bool store_ok(int s) { return s >= TLBM_STORE_F32 && s <= TLBM_STORE_BF16_DEV; }
 
+size_t store_size(int s) {
+  switch (s) {
+    case TLBM_STORE_F32:
+      return 4;
+    case TLBM_STORE_F64:
+      return 8;
+    default:
+      return 2;
+  }
+}
+
 int check_dims(int lattice, int nz, int ny, int nx) {
   if (!lattice_ok(lattice)) return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", lattice);
   if (nz < 1 || ny < 1 || nx < 1) return fail(TLBM_ERR_SHAPE, "grid dims must be >= 1 (got %d,%d,%d)", nz, ny, nx);
@@ -347,16 +413,45 @@ int check_dims(int lattice, int nz, int ny, int nx) {
   return TLBM_OK;
 }
 
+// strides (q, z, y) in elements, x stride 1; NULL = contiguous [q][z][y][x]
+int resolve_strides(const int64_t* s, int lattice, int nz, int ny, int nx, size_t esize, int64_t out[3],
+                    const char* which) {
+  if (s == nullptr) {
+    out[0] = int64_t(nz) * ny * nx;
+    out[1] = int64_t(ny) * nx;
+    out[2] = nx;
+  } else {
+    out[0] = s[0];
+    out[1] = s[1];
+    out[2] = s[2];
+  }
+  if (out[0] <= 0 || out[2] < nx || (nz > 1 && out[1] <= 0))
+    return fail(TLBM_ERR_SHAPE, "%s strides (%lld, %lld, %lld) are not a valid population layout", which,
+                (long long)out[0], (long long)out[1], (long long)out[2]);
+  // per-lane offsets y*sy + x must stay below 2^32 bytes
+  const uint64_t span = (uint64_t(ny - 1) * uint64_t(out[2]) + uint64_t(nx)) * esize;
+  if (span >= (uint64_t(1) << 32))
+    return fail(TLBM_ERR_SHAPE, "%s: one z-plane spans %llu bytes; the per-lane offset limit is 4 GiB", which,
+                (unsigned long long)span);
+  (void)lattice;
+  return TLBM_OK;
+}
+
 template <typename C>
-int fill_args(KArgs<C>& a, int lattice, const void* fin, void* fout, int nz, int ny, int nx,
-              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
-              int* bx_out, int* by_out, uint32_t* nblocks) {
+int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void* fout, const int64_t* os, int nz,
+              int ny, int nx, const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
+              double* force, int* bx_out, int* by_out, uint32_t* nblocks) {
   memset(&a, 0, sizeof(a));
   a.fin = fin;
   a.fout = fout;
   a.solid = solid;
   a.force = force;
-  a.N = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  a.isq = is[0];
+  a.isz = is[1];
+  a.isy = is[2];
+  a.osq = os[0];
+  a.osz = os[1];
+  a.osy = os[2];
   a.nx = nx;
   a.ny = ny;
   a.nz = nz;
@@ -389,12 +484,11 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, void* fout, int nz, int
     if (coll->tau_mode == TLBM_TAU_FIELD && coll->tau_eff == nullptr)
       return fail(TLBM_ERR_INVALID, "TLBM_TAU_FIELD needs tau_eff");
     if (coll->tau_mode == TLBM_TAU_MLP)
-      return fail(TLBM_ERR_UNSUPPORTED, "TLBM_TAU_MLP is served by tlbm_step_mlp");
+      return fail(TLBM_ERR_UNSUPPORTED, "TLBM_TAU_MLP is served by the closure entry points");
     if (coll->kind == TLBM_COLL_MRT) {
       // MRT rate vector; negative marks the shear moments (rate = omega)
       const double se = coll->rate[0], seps = coll->rate[1], sq = coll->rate[2], spi = coll->rate[3];
       double s[27];
-      int q = lattice;
       for (int i = 0; i < 27; ++i) s[i] = 0.0;
       if (lattice == TLBM_D2Q9) {
         const double v[9] = {0, se, seps, 0, sq, 0, sq, -1, -1};
@@ -405,7 +499,7 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, void* fout, int nz, int
       } else {
         for (int i = 4; i < 27; ++i) s[i] = i == 4 ? se : (i < 10 ? -1 : (i < 19 ? sq : (i == 19 ? seps : spi)));
       }
-      for (int i = 0; i < q; ++i) a.s[i] = C(s[i]);
+      for (int i = 0; i < lattice; ++i) a.s[i] = C(s[i]);
     }
   }
   a.bc_kind = TLBM_BC_NONE;
@@ -440,24 +534,38 @@ constexpr bool coll_built() {
   return false;
 }
 
+struct Call {
+  int lattice, store, kind;
+  bool stream_, feat;
+  const void* fin;
+  const int64_t* is;
+  void* fout;
+  const int64_t* os;
+  int nz, ny, nx;
+  const tlbm_collision* coll;
+  const int8_t* solid;
+  const tlbm_boundary* bc;
+  double* force;
+  hipStream_t s;
+};
+
 template <class L, int S, int COLL>
-int run_coll(bool stream_, bool feat, int lattice, const void* fin, void* fout, int nz, int ny, int nx,
-             const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
-             hipStream_t s) {
+int run_coll(const Call& c) {
   if constexpr (!coll_built<L, COLL>()) {
     return fail(TLBM_ERR_UNSUPPORTED, "collision %d is not built for Q=%d", COLL, L::Q);
   } else {
     KArgs<typename Store<S>::C> a;
     int bx, by;
     uint32_t nb;
-    int rc = fill_args(a, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, &bx, &by, &nb);
+    int rc = fill_args(a, c.lattice, c.fin, c.is, c.fout, c.os, c.nz, c.ny, c.nx, c.coll, c.solid, c.bc, c.force, &bx,
+                       &by, &nb);
     if (rc != TLBM_OK) return rc;
-    if (stream_) {
-      if (feat) return launch_lbm<L, S, COLL, true, FEAT_ALL>(a, bx, by, nb, s);
-      return launch_lbm<L, S, COLL, true, FEAT_NONE>(a, bx, by, nb, s);
+    if (c.stream_) {
+      if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_ALL>(a, bx, by, nb, c.s);
+      return launch_lbm<L, S, COLL, true, FEAT_NONE>(a, bx, by, nb, c.s);
     }
     if constexpr (S == ST_F32 || S == ST_F64) {
-      return launch_lbm<L, S, COLL, false, FEAT_NONE>(a, bx, by, nb, s);
+      return launch_lbm<L, S, COLL, false, FEAT_NONE>(a, bx, by, nb, c.s);
     } else {
       return fail(TLBM_ERR_DTYPE, "collide-only ops take float32 or float64 populations");
     }
@@ -465,60 +573,55 @@ int run_coll(bool stream_, bool feat, int lattice, const void* fin, void* fout, 
 }
 
 template <class L, int S>
-int run_store(int kind, bool stream_, bool feat, int lattice, const void* fin, void* fout, int nz, int ny,
-              int nx, const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
-              double* force, hipStream_t s) {
-  switch (kind) {
-#define TLBM_CASE(K) \
-  case K:            \
-    return run_coll<L, S, K>(stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, s);
-    TLBM_CASE(TLBM_COLL_NONE)
-    TLBM_CASE(TLBM_COLL_BGK)
-    TLBM_CASE(TLBM_COLL_MRT)
-    TLBM_CASE(TLBM_COLL_CM)
-    TLBM_CASE(TLBM_COLL_CUMULANT)
-    TLBM_CASE(TLBM_COLL_GEIER)
-#undef TLBM_CASE
+int run_store(const Call& c) {
+  switch (c.kind) {
+    case TLBM_COLL_NONE:
+      return run_coll<L, S, TLBM_COLL_NONE>(c);
+    case TLBM_COLL_BGK:
+      return run_coll<L, S, TLBM_COLL_BGK>(c);
+    case TLBM_COLL_MRT:
+      return run_coll<L, S, TLBM_COLL_MRT>(c);
+    case TLBM_COLL_CM:
+      return run_coll<L, S, TLBM_COLL_CM>(c);
+    case TLBM_COLL_CUMULANT:
+      return run_coll<L, S, TLBM_COLL_CUMULANT>(c);
+    case TLBM_COLL_GEIER:
+      return run_coll<L, S, TLBM_COLL_GEIER>(c);
     default:
-      return fail(TLBM_ERR_INVALID, "unknown collision kind %d", kind);
+      return fail(TLBM_ERR_INVALID, "unknown collision kind %d", c.kind);
   }
 }
 
 template <class L>
-int run_lattice(int store, int kind, bool stream_, bool feat, int lattice, const void* fin, void* fout,
-                int nz, int ny, int nx, const tlbm_collision* coll, const int8_t* solid,
-                const tlbm_boundary* bc, double* force, hipStream_t s) {
-  switch (store) {
-#define TLBM_CASE(S) \
-  case S:            \
-    return run_store<L, S>(kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc, force, s);
-    TLBM_CASE(ST_F32)
-    TLBM_CASE(ST_F64)
-    TLBM_CASE(ST_F16)
-    TLBM_CASE(ST_BF16)
-    TLBM_CASE(ST_F16_DEV)
-    TLBM_CASE(ST_BF16_DEV)
-#undef TLBM_CASE
+int run_lattice(const Call& c) {
+  switch (c.store) {
+    case ST_F32:
+      return run_store<L, ST_F32>(c);
+    case ST_F64:
+      return run_store<L, ST_F64>(c);
+    case ST_F16:
+      return run_store<L, ST_F16>(c);
+    case ST_BF16:
+      return run_store<L, ST_BF16>(c);
+    case ST_F16_DEV:
+      return run_store<L, ST_F16_DEV>(c);
+    case ST_BF16_DEV:
+      return run_store<L, ST_BF16_DEV>(c);
     default:
-      return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+      return fail(TLBM_ERR_DTYPE, "unknown storage %d", c.store);
   }
 }
 
-int run(int lattice, int store, int kind, bool stream_, bool feat, const void* fin, void* fout, int nz,
-        int ny, int nx, const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc,
-        double* force, hipStream_t s) {
-  switch (lattice) {
+int run(const Call& c) {
+  switch (c.lattice) {
     case TLBM_D2Q9:
-      return run_lattice<D2Q9>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
-                               force, s);
+      return run_lattice<D2Q9>(c);
     case TLBM_D3Q19:
-      return run_lattice<D3Q19>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
-                                force, s);
+      return run_lattice<D3Q19>(c);
     case TLBM_D3Q27:
-      return run_lattice<D3Q27>(store, kind, stream_, feat, lattice, fin, fout, nz, ny, nx, coll, solid, bc,
-                                force, s);
+      return run_lattice<D3Q27>(c);
     default:
-      return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", lattice);
+      return fail(TLBM_ERR_INVALID, "unknown lattice Q=%d", c.lattice);
   }
 }
 
@@ -558,11 +661,22 @@ int by_store(int store, F&& fn) {
 
 uint32_t blocks_for(uint64_t n) { return uint32_t((n + 255) / 256); }
 
+int prepare(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+            const int64_t* out_strides, int nz, int ny, int nx, int64_t is[3], int64_t os[3]) {
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
+  rc = resolve_strides(in_strides, lattice, nz, ny, nx, store_size(store), is, "f_in");
+  if (rc) return rc;
+  return resolve_strides(out_strides, lattice, nz, ny, nx, store_size(store), os, "f_out");
+}
+
 }  // namespace
 
 extern "C" {
 
-int tlbm_version(void) { return 1; }
+int tlbm_version(void) { return 2; }
 
 const char* tlbm_last_error(void) { return g_err; }
 
@@ -587,45 +701,47 @@ const char* tlbm_status_string(int status) {
   }
 }
 
-int tlbm_step(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-              const tlbm_collision* coll, const int8_t* solid, const tlbm_boundary* bc, double* force,
-              tlbm_stream_t stream) {
+int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+              const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll, const int8_t* solid,
+              const tlbm_boundary* bc, double* force, tlbm_stream_t stream) {
   g_err[0] = 0;
-  int rc = check_dims(lattice, nz, ny, nx);
+  int64_t is[3], os[3];
+  int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
   if (rc) return rc;
-  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
-  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
   if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
   const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
   if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
-  return run(lattice, store, coll->kind, true, feat, f_in, f_out, nz, ny, nx, coll, solid, bc, force,
-             (hipStream_t)stream);
+  Call c{lattice, store, coll->kind, true, feat, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
+         (hipStream_t)stream};
+  return run(c);
 }
 
-int tlbm_stream(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-                tlbm_stream_t stream) {
+int tlbm_stream(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+                const int64_t* out_strides, int nz, int ny, int nx, tlbm_stream_t stream) {
   g_err[0] = 0;
-  int rc = check_dims(lattice, nz, ny, nx);
+  int64_t is[3], os[3];
+  int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
   if (rc) return rc;
-  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
-  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; streaming needs two buffers");
-  return run(lattice, store, TLBM_COLL_NONE, true, false, f_in, f_out, nz, ny, nx, nullptr, nullptr, nullptr,
-             nullptr, (hipStream_t)stream);
+  Call c{lattice, store, TLBM_COLL_NONE, true, false, f_in, is, f_out, os, nz, ny, nx, nullptr, nullptr, nullptr,
+         nullptr, (hipStream_t)stream};
+  return run(c);
 }
 
-int tlbm_collide(int lattice, int store, const void* f_in, void* f_out, int nz, int ny, int nx,
-                 const tlbm_collision* coll, tlbm_stream_t stream) {
+int tlbm_collide(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+                 const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,
+                 tlbm_stream_t stream) {
   g_err[0] = 0;
-  int rc = check_dims(lattice, nz, ny, nx);
-  if (rc) return rc;
   if (store != TLBM_STORE_F32 && store != TLBM_STORE_F64)
     return fail(TLBM_ERR_DTYPE, "collide-only ops take float32 or float64 populations");
-  if (f_in == nullptr || f_out == nullptr) return fail(TLBM_ERR_INVALID, "null population pointer");
+  int64_t is[3], os[3];
+  int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
+  if (rc) return rc;
   if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return fail(TLBM_ERR_INVALID, "collision kind required");
-  return run(lattice, store, coll->kind, false, false, f_in, f_out, nz, ny, nx, coll, nullptr, nullptr, nullptr,
-             (hipStream_t)stream);
+  Call c{lattice, store, coll->kind, false, false, f_in, is, f_out, os, nz, ny, nx, coll, nullptr, nullptr, nullptr,
+         (hipStream_t)stream};
+  return run(c);
 }
 
 int tlbm_equilibrium(int lattice, int dtype, const void* rho, const void* ux, const void* uy, const void* uz,
@@ -698,48 +814,60 @@ int tlbm_bounce_back(int lattice, int dtype, const void* f_in, void* f_out, cons
   });
 }
 
-static int launch_sums(int lattice, int store, const void* f, const int8_t* solid, int64_t ncells, double* out,
-                       hipStream_t s) {
-  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
-  if (ncells == 0) return TLBM_OK;
+static int launch_sums(int lattice, int store, const void* f, const int64_t* strides, int nz, int ny, int nx,
+                       const int8_t* solid, double* out, hipStream_t s) {
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
   if (!f || !out) return fail(TLBM_ERR_INVALID, "null pointer");
-  const uint64_t n = uint64_t(ncells);
+  int64_t st[3];
+  rc = resolve_strides(strides, lattice, nz, ny, nx, store_size(store), st, "f");
+  if (rc) return rc;
+  const Layout lay{st[0], st[1], st[2], nx, ny, nz};
+  const uint64_t n = uint64_t(nz) * ny * nx;
   uint32_t nb = blocks_for(n);
   if (nb > 4096) nb = 4096;
   return by_lattice(lattice, [&](auto lat) -> int {
     using L = decltype(lat);
     return by_store<L>(store, [&](auto sk) -> int {
       constexpr int S = decltype(sk)::value;
-      hipLaunchKernelGGL((k_moment_sums<L, S>), dim3(nb), dim3(256), 0, s, f, solid, n, out);
+      hipLaunchKernelGGL((k_moment_sums<L, S>), dim3(nb), dim3(256), 0, s, f, lay, solid, out);
       return check_launch("k_moment_sums");
     });
   });
 }
 
-int tlbm_moment_sums(int lattice, int store, const void* f, int64_t ncells, double* out, tlbm_stream_t stream) {
+int tlbm_moment_sums(int lattice, int store, const void* f, const int64_t* strides, int nz, int ny, int nx,
+                     double* out, tlbm_stream_t stream) {
   g_err[0] = 0;
-  return launch_sums(lattice, store, f, nullptr, ncells, out, (hipStream_t)stream);
+  return launch_sums(lattice, store, f, strides, nz, ny, nx, nullptr, out, (hipStream_t)stream);
 }
 
-int tlbm_obstacle_force(int lattice, int store, const void* f, const int8_t* solid, int64_t ncells, double* out,
-                        tlbm_stream_t stream) {
+int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* strides, int nz, int ny, int nx,
+                        const int8_t* solid, double* out, tlbm_stream_t stream) {
   g_err[0] = 0;
   if (!solid) return fail(TLBM_ERR_INVALID, "null solid mask");
-  return launch_sums(lattice, store, f, solid, ncells, out, (hipStream_t)stream);
+  return launch_sums(lattice, store, f, strides, nz, ny, nx, solid, out, (hipStream_t)stream);
 }
 
-int tlbm_scale(int lattice, int store, void* f, int64_t ncells, const double* scale, tlbm_stream_t stream) {
+int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx, const double* scale,
+               tlbm_stream_t stream) {
   g_err[0] = 0;
-  if (ncells < 0) return fail(TLBM_ERR_SHAPE, "negative cell count");
-  if (ncells == 0) return TLBM_OK;
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
   if (!f || !scale) return fail(TLBM_ERR_INVALID, "null pointer");
+  int64_t st[3];
+  rc = resolve_strides(strides, lattice, nz, ny, nx, store_size(store), st, "f");
+  if (rc) return rc;
+  const Layout lay{st[0], st[1], st[2], nx, ny, nz};
   hipStream_t s = (hipStream_t)stream;
-  const uint64_t n = uint64_t(ncells);
+  const uint64_t n = uint64_t(nz) * ny * nx;
   return by_lattice(lattice, [&](auto lat) -> int {
     using L = decltype(lat);
     return by_store<L>(store, [&](auto sk) -> int {
       constexpr int S = decltype(sk)::value;
-      hipLaunchKernelGGL((k_scale<L, S>), dim3(blocks_for(n)), dim3(256), 0, s, f, n, scale);
+      hipLaunchKernelGGL((k_scale<L, S>), dim3(blocks_for(n)), dim3(256), 0, s, f, lay, scale);
       return check_launch("k_scale");
     });
   });

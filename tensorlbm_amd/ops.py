@@ -57,12 +57,14 @@ def _grid(f: torch.Tensor, q: int, what: str) -> tuple[int, int, int]:
     raise ValueError(f"{what}: populations must have shape {want}, got {tuple(f.shape)}")
 
 
-def _pops(f: torch.Tensor, q: int, what: str) -> tuple[torch.Tensor, tuple[int, int, int]]:
+def _pops(f: torch.Tensor, q: int, what: str, contiguous: bool = True) -> tuple[torch.Tensor, tuple[int, int, int]]:
     nat.require_gpu(f, what)
     if f.shape[0] != q:
         raise ValueError(f"{what}: expected Q={q} populations, got shape {tuple(f.shape)}")
     dims = _grid(f, q, what)
-    return f.contiguous(), dims
+    if contiguous or f.stride(-1) != 1:
+        f = f.contiguous()
+    return f, dims
 
 
 # ---------------------------------------------------------------------------
@@ -158,7 +160,7 @@ def _collide(f, q, coll, what):
     f, dims = _pops(f, q, what)
     dt = _eager_dtype(f, what)
     out = torch.empty_like(f)
-    rc = nat.lib().tlbm_collide(q, dt, f.data_ptr(), out.data_ptr(), *dims, coll, nat.stream_of(f))
+    rc = nat.lib().tlbm_collide(q, dt, f.data_ptr(), None, out.data_ptr(), None, *dims, coll, nat.stream_of(f))
     nat.check(rc, what)
     return out
 
@@ -300,10 +302,11 @@ def collide_smagorinsky_bgk27(f, tau, C_s=0.1):
 
 
 def _stream(f, q, what):
-    f, dims = _pops(f, q, what)
+    f, dims = _pops(f, q, what, contiguous=False)
     store = nat.store_code(f.dtype)
-    out = torch.empty_like(f)
-    rc = nat.lib().tlbm_stream(q, store, f.data_ptr(), out.data_ptr(), *dims, nat.stream_of(f))
+    out = torch.empty(f.shape, dtype=f.dtype, device=f.device)
+    rc = nat.lib().tlbm_stream(q, store, f.data_ptr(), nat.strides_of(f), out.data_ptr(), None, *dims,
+                               nat.stream_of(f))
     nat.check(rc, what)
     return out
 
@@ -361,12 +364,15 @@ def bounce_back_cells_27(f, mask):
 
 
 def moment_sums(f, *, deviation: bool = False) -> torch.Tensor:
-    """FP64 ``(Σf, Σc_x f, Σc_y f, Σc_z f)`` over the whole lattice, one launch, device tensor."""
+    """FP64 ``(Σf, Σc_x f, Σc_y f, Σc_z f)`` over the whole lattice, one launch, device tensor.
+
+    Any storage dtype and layout; ``deviation=True`` reads 16-bit ``f_i − w_i`` storage.
+    """
     q = int(f.shape[0])
-    f, dims = _pops(f, q, "moment_sums")
+    f, dims = _pops(f, q, "moment_sums", contiguous=False)
     out = torch.zeros(4, dtype=torch.float64, device=f.device)
-    rc = nat.lib().tlbm_moment_sums(q, nat.store_code(f.dtype, deviation), f.data_ptr(),
-                                    dims[0] * dims[1] * dims[2], out.data_ptr(), nat.stream_of(f))
+    rc = nat.lib().tlbm_moment_sums(q, nat.store_code(f.dtype, deviation), f.data_ptr(), nat.strides_of(f), *dims,
+                                    out.data_ptr(), nat.stream_of(f))
     nat.check(rc, "moment_sums")
     return out
 
@@ -374,13 +380,13 @@ def moment_sums(f, *, deviation: bool = False) -> torch.Tensor:
 def compute_obstacle_forces_3d(f, obstacle_mask):
     """Ladd momentum-exchange force ``2 Σ_solid Σ_q c_q f`` (obstacles.py:91-130), FP64 accumulated."""
     q = int(f.shape[0])
-    f, dims = _pops(f, q, "compute_obstacle_forces_3d")
+    f, dims = _pops(f, q, "compute_obstacle_forces_3d", contiguous=False)
     if tuple(obstacle_mask.shape) != tuple(f.shape[1:]):
         raise ValueError("obstacle_mask shape must match the population spatial shape")
     m = obstacle_mask.to(device=f.device, dtype=torch.int8).contiguous()
     acc = torch.zeros(3, dtype=torch.float64, device=f.device)
-    rc = nat.lib().tlbm_obstacle_force(q, nat.store_code(f.dtype), f.data_ptr(), m.data_ptr(),
-                                       dims[0] * dims[1] * dims[2], acc.data_ptr(), nat.stream_of(f))
+    rc = nat.lib().tlbm_obstacle_force(q, nat.store_code(f.dtype), f.data_ptr(), nat.strides_of(f), *dims,
+                                       m.data_ptr(), acc.data_ptr(), nat.stream_of(f))
     nat.check(rc, "compute_obstacle_forces_3d")
     res = acc.to(f.dtype)
     return res[0], res[1], res[2]
@@ -390,12 +396,11 @@ def _correct(f, q, target_mass, what):
     f, dims = _pops(f, q, what)
     store = nat.store_code(f.dtype)
     sums = moment_sums(f)
-    n = dims[0] * dims[1] * dims[2]
     # scale = target / current, or 1 when the current mass vanishes (solver.py:433-436)
     cur = sums[0:1]
     scale = torch.where(cur.abs() < 1e-30, torch.ones_like(cur), float(target_mass) / cur)
     out = f.clone()
-    rc = nat.lib().tlbm_scale(q, store, out.data_ptr(), n, scale.data_ptr(), nat.stream_of(f))
+    rc = nat.lib().tlbm_scale(q, store, out.data_ptr(), None, *dims, scale.data_ptr(), nat.stream_of(f))
     nat.check(rc, what)
     return out
 

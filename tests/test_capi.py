@@ -38,18 +38,25 @@ def test_argument_validation_happens_before_any_launch():
     """Validation paths return error codes without touching a device."""
     lib = nat.lib()
     coll = nat.make_collision(nat.COLL_BGK, 0.7)
-    p = ctypes.c_void_p(0x1000)
-    assert lib.tlbm_step(19, 0, p, p, 4, 4, 4, coll, None, None, None, None) == nat.ERR_ALIAS
-    assert lib.tlbm_step(19, 0, None, p, 4, 4, 4, coll, None, None, None, None) == nat.ERR_INVALID
-    assert lib.tlbm_step(20, 0, p, ctypes.c_void_p(0x2000), 4, 4, 4, coll, None, None, None, None) == nat.ERR_INVALID
-    assert lib.tlbm_step(9, 0, p, ctypes.c_void_p(0x2000), 2, 4, 4, coll, None, None, None, None) == nat.ERR_SHAPE
-    assert lib.tlbm_step(19, 9, p, ctypes.c_void_p(0x2000), 4, 4, 4, coll, None, None, None, None) == nat.ERR_DTYPE
-    bad = nat.make_collision(nat.COLL_BGK, 0.4)
-    assert lib.tlbm_step(19, 0, p, ctypes.c_void_p(0x2000), 4, 4, 4, bad, None, None, None, None) == nat.ERR_INVALID
-    assert lib.tlbm_stream(19, 0, p, p, 4, 4, 4, None) == nat.ERR_ALIAS
-    assert lib.tlbm_collide(19, 2, p, p, 4, 4, 4, coll, None) == nat.ERR_DTYPE
+    p, p2 = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+
+    def step(q=19, store=0, a=p, b=p2, dims=(4, 4, 4), c=coll, si=None, so=None):
+        return lib.tlbm_step(q, store, a, si, b, so, *dims, c, None, None, None, None)
+
+    assert step(b=p) == nat.ERR_ALIAS
+    assert step(a=None) == nat.ERR_INVALID
+    assert step(q=20) == nat.ERR_INVALID
+    assert step(q=9, dims=(2, 4, 4)) == nat.ERR_SHAPE
+    assert step(store=9) == nat.ERR_DTYPE
+    assert step(c=nat.make_collision(nat.COLL_BGK, 0.4)) == nat.ERR_INVALID
     big = 1 << 11
-    assert lib.tlbm_step(19, 0, p, ctypes.c_void_p(0x2000), big, big, big, coll, None, None, None, None) == nat.ERR_SHAPE
+    assert step(dims=(big, big, big)) == nat.ERR_SHAPE
+    assert step(si=nat.Strides(64, 16, 2)) == nat.ERR_SHAPE  # y stride below nx
+    # a z-plane spanning more than 4 GiB cannot use 32-bit lane offsets
+    assert step(dims=(1, 40000, 40000), si=nat.Strides(1 << 40, 1 << 40, 40000)) in (nat.ERR_SHAPE,)
+    assert lib.tlbm_stream(19, 0, p, None, p, None, 4, 4, 4, None) == nat.ERR_ALIAS
+    assert lib.tlbm_collide(19, 2, p, None, p, None, 4, 4, 4, coll, None) == nat.ERR_DTYPE
+    assert b"float32 or float64" in lib.tlbm_last_error()
 
 
 def test_check_maps_status_to_reference_exceptions():

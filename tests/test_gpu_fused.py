@@ -191,3 +191,38 @@ def test_moment_sums_and_correct_mass(dev):
     out = tl.correct_mass3d(fg, 100.0)
     assert abs(float(out.sum()) - 100.0) < 1e-9
     assert np.allclose(_np(out), ref.correct_mass(f0, 100.0), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# layouts: the solver's padded [z][q][y][x] buffers vs contiguous SoA
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("q,shape,dtype", [(19, (10, 12, 130), torch.float32), (27, (6, 8, 10), torch.float64),
+                                           (9, (1, 30, 70), torch.float32), (19, (8, 8, 8), torch.float16)])
+def test_padded_layout_bit_identical_to_contiguous(q, shape, dtype, dev):
+    f0 = F.random_state(q, shape, seed=21, noise=5e-3)
+    if q == 9:
+        f0 = f0[:, 0]
+    lat = {9: "D2Q9", 19: "D3Q19", 27: "D3Q27"}[q]
+    nz, ny, nx = shape
+    a = tl.FusedSolver3D(nz, ny, nx, tau=0.66, device=dev, dtype=dtype, lattice=lat)
+    b = tl.FusedSolver3D(nz, ny, nx, tau=0.66, device=dev, dtype=dtype, lattice=lat, layout="contiguous")
+    fa = fb = _gpu(f0, dev, dtype)
+    for _ in range(6):
+        fa = a.step(fa)
+        fb = b.step(fb)
+    assert not fa.is_contiguous() and fb.is_contiguous()
+    assert torch.equal(fa, fb)
+    assert torch.equal(tl.moment_sums(fa), tl.moment_sums(fb))
+
+
+def test_stream_between_layouts(dev):
+    f = _gpu(F.random_state(19, (5, 6, 7), seed=4), dev)
+    padded = tl.fused.padded_populations(19, 5, 6, 7, torch.float32, dev)
+    padded.copy_(f)
+    got = tl.hip_stream(padded)
+    assert torch.equal(got, tl.stream3d(f))
+    out = tl.fused.padded_populations(19, 5, 6, 7, torch.float32, dev)
+    tl.hip_stream(f, out=out)
+    assert torch.equal(out, tl.stream3d(f))
