@@ -136,7 +136,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              "(the per-step update is single-GPU here; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the per-step update runs on one GPU; a z-slab split with one halo exchange per step is the "
+              "multi-GPU design, not built)", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -61,7 +61,8 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
 #define TLBM_TAU_SMAG 1  /* clamped Hou closure, turbulence.py:183-208 */
 #define TLBM_TAU_FIELD 2 /* per-cell tau_eff array, triton_fused_obstacle.py:1300-1312 */
 #define TLBM_TAU_SMAG_RAW 3 /* unclamped Hou closure of cumulant.py:1316-1321 */
-#define TLBM_TAU_MLP 4   /* learned eddy-viscosity closure (tlbm_mlp_params) */
+#define TLBM_TAU_MLP 4   /* learned closure: tau_eff = max(tau + 3 nu_t, 0.501) (ai/inference.py:56-65),
+                            nu_t from the tlbm_mlp_pack blob in coll->mlp (D3Q19 BGK) */
 
 /* boundary kinds for tlbm_step */
 #define TLBM_BC_NONE 0
@@ -75,7 +76,9 @@ typedef struct tlbm_collision {
   int32_t eq_rho_floor; /* 1: f_eq built with max(rho,1e-12) (fused kernels); 0: with rho */
   int32_t reserved;
   double tau;           /* molecular relaxation time */
-  double cs_delta_sq;   /* Cs^2 Delta^2 for the Smagorinsky modes */
+  double cs_delta_sq;   /* Cs^2 Delta^2 for the Smagorinsky modes; TLBM_TAU_MLP: the strain
+                           features use the Smagorinsky tau of this Cs as time scale
+                           (0: the molecular tau, _strain_from_fneq_3d, turbulence.py:1449-1480) */
   double rate[8];       /* MRT: s_e, s_eps, s_q, s_pi | CM: s_bulk, s_3, s_4, s_5, s_6 |
                            CUMULANT/GEIER: omega_b, omega_odd, omega_even */
   const float* tau_eff; /* TLBM_TAU_FIELD: (nz,ny,nx) float32 */
@@ -151,6 +154,47 @@ int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* st
 /* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
 int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx,
                const double* scale, tlbm_stream_t stream);
+
+/*
+ * Learned eddy-viscosity closure, 3-D 7 -> 32 -> 32 -> 1 MLP (ReLU, Softplus(beta=10) output).
+ * The reference's closure is 2-D only (EddyViscosityMLP, src/tensorlbm/ai/model.py:19-92,
+ * trained by train_eddy_viscosity_model, ai/train.py:310-464, applied by collide_ai_les_bgk,
+ * ai/inference.py:68-92); this 3-D form has no counterpart there.  Parameters are one fp32
+ * vector of ntotal floats: W1[32][7] b1[32] W2[32][32] b2[32] w3[32] b3 | mean[7] inv_std[7]
+ * (the first nparam are trainable, the last 14 normalise the inputs).
+ * Features: S_xx S_yy S_zz S_xy S_xz S_yz |S| of the resolved strain rate; output nu_t >= 0.
+ */
+/* sizes of the parameter vector, the packed matrix-core blob and the gradient block */
+int tlbm_mlp_sizes(int64_t* nparam, int64_t* ntotal, int64_t* packed_bytes, int64_t* batch_block);
+
+/* fp32 parameters -> packed bf16 blob used by tlbm_step with TLBM_TAU_MLP (coll->mlp) */
+int tlbm_mlp_pack(const float* params, void* packed, tlbm_stream_t stream);
+
+/* y[n] = MLP(x[n][7]) in fp32 (predict_tau_eff_2d's network call, ai/inference.py:53-65) */
+int tlbm_mlp_predict(const float* params, const float* x, int64_t n, float* y, tlbm_stream_t stream);
+
+/* MSE gradient partials: partials[ceil(n / batch_block)][nparam + 1] (last column: loss share) */
+int tlbm_mlp_grad(const float* params, const float* x, const float* t, int64_t n, float* partials,
+                  tlbm_stream_t stream);
+
+/* sum the partials in a fixed order, clip the gradient's global 2-norm to clip_norm
+ * (<= 0: no clipping; clip_grad_norm_ in ai/train.py:401-405) and apply one Adam step
+ * (torch.optim.Adam rule, ai/train.py:382-435).  lr = 0 only reduces.  loss_out (MSE) and
+ * grad_out (the unclipped gradient, nparam floats) may be NULL. */
+int tlbm_mlp_adam(float* params, const float* partials, int nblocks, float* m, float* v, int step,
+                  float lr, float beta1, float beta2, float eps, float clip_norm, float* loss_out,
+                  float* grad_out, tlbm_stream_t stream);
+
+/* block average by an integer factor (_coarsen_mean, ai/pipeline.py:93-107, in 3-D):
+ * out[10][nz/f][ny/f][nx/f] = rho, u_x, u_y, u_z, <uxux>, <uyuy>, <uzuz>, <uxuy>, <uxuz>, <uyuz> */
+int tlbm_coarsen_moments(int lattice, int store, const void* f, const int64_t* strides, int nz,
+                         int ny, int nx, int factor, float* out, tlbm_stream_t stream);
+
+/* training samples from coarsened moments: periodic central-difference strain
+ * (strain_rate_tensor_2d, ai/dataset.py:41-72, in 3-D) as feats[n][7] and
+ * nu = max(0, -tau_ij S_ij / (2 S_kl S_kl)), tau_ij = <u_i u_j> - <u_i><u_j>, as labels[n] */
+int tlbm_sgs_samples(const float* fields, int nz, int ny, int nx, float* feats, float* labels,
+                     tlbm_stream_t stream);
 
 #ifdef __cplusplus
 }

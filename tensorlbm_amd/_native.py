@@ -73,6 +73,14 @@ _SIGNATURES = {
     "tlbm_moment_sums": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
     "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_scale": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_mlp_sizes": ([_VP, _VP, _VP, _VP], _I),
+    "tlbm_mlp_pack": ([_VP, _VP, _VP], _I),
+    "tlbm_mlp_predict": ([_VP, _VP, _I64, _VP, _VP], _I),
+    "tlbm_mlp_grad": ([_VP, _VP, _VP, _I64, _VP, _VP], _I),
+    "tlbm_mlp_adam": ([_VP, _VP, _I, _VP, _VP, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                       ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
+    "tlbm_coarsen_moments": ([_I, _I, _VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_sgs_samples": ([_VP, _I, _I, _I, _VP, _VP, _VP], _I),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -34,7 +34,8 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "tlbm.hip")]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", "-o", str(tmp), str(CSRC / "tlbm.hip"),
+           str(CSRC / "closure.hip")]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

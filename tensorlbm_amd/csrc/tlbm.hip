@@ -31,11 +31,14 @@
 
 #include "../../include/tlbm_hip.h"
 #include "collide.hpp"
+#include "mlp.hpp"
 #include "store.hpp"
 
 namespace tlbm {
 
-enum Feat { FEAT_NONE = 0, FEAT_ALL = 1 };
+// FEAT_ALL: solid mask / force / boundaries; FEAT_MLP: learned closure (whole
+// waves stay alive for the matrix-core evaluation).  Bits combine.
+enum Feat { FEAT_NONE = 0, FEAT_ALL = 1, FEAT_MLP = 2, FEAT_MLP_ALL = 3 };
 
 template <typename C>
 struct KArgs {
@@ -90,12 +93,25 @@ __device__ __forceinline__ typename ST::C pull_one(const typename ST::T* fin, co
   return ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
 }
 
+extern __shared__ __align__(16) unsigned char dyn_lds[];
+
 template <class L, int S, int COLL, bool STREAM, int FEAT>
 __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
   using ST = Store<S>;
   using T = typename ST::T;
   using C = typename ST::C;
   constexpr int Q = L::Q;
+  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
+  constexpr bool kMlp = (FEAT & FEAT_MLP) != 0;
+
+  if constexpr (kMlp) {
+    // stage the matrix-core operand blob in LDS (4544 B = 284 x 16 B)
+    const uint4* src = static_cast<const uint4*>(a.mlp);
+    uint4* dst = reinterpret_cast<uint4*>(dyn_lds);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    for (int i = tid; i < int(sizeof(mlp::Packed) / 16); i += 256) dst[i] = src[i];
+    __syncthreads();
+  }
 
   const uint32_t tile = blockIdx.x;
   const uint32_t txi = tile % uint32_t(a.tiles_x);
@@ -105,7 +121,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
   const int x = int(txi * blockDim.x + threadIdx.x);
   const int y = int(tyi * blockDim.y + threadIdx.y);
   const bool active = x < a.nx && y < a.ny;
-  if (FEAT != FEAT_ALL && !active) return;
+  if (FEAT == FEAT_NONE && !active) return;
   // inactive lanes (force path only) work on cell (0, 0, z): in bounds, never stored
   const int xx = active ? x : 0;
   const int yy = active ? y : 0;
@@ -134,7 +150,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
 
   bool solid = false;
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  if constexpr (FEAT == FEAT_ALL) {
+  if constexpr (kAll) {
     if (a.solid != nullptr) solid = active && a.solid[cell] != 0;
     if (a.force != nullptr && solid) {
       // Ladd sample: post-stream, pre-bounce-back (obstacles.py:91-130)
@@ -178,7 +194,27 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
     C omega = a.omega;
     const bool need_pi = (COLL == TLBM_COLL_CUMULANT) || a.tau_mode == TLBM_TAU_SMAG ||
                          a.tau_mode == TLBM_TAU_SMAG_RAW;
-    if constexpr (COLL == TLBM_COLL_BGK) {
+    if constexpr (COLL == TLBM_COLL_BGK && kMlp) {
+      // learned closure: strain from the non-equilibrium stress with the
+      // Smagorinsky tau as reference time scale, S = -Pi / (2 rho cs^2 tau_ref)
+      C feq[Q], fneq[Q], pi[6];
+      equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
+      neq_stress<L>(fneq, pi);
+      const C om_ref = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
+      const C sc = C(-1.5) * om_ref / mc.rho_safe;
+      float xs[7];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) xs[k] = float(sc * pi[k]);
+      xs[6] = sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
+                            2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
+      const float nu = mlp::nu_t(reinterpret_cast<const mlp::Packed*>(dyn_lds), xs);
+      // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
+      C te = a.tau + C(3) * C(nu);
+      te = te < C(0.501) ? C(0.501) : te;
+      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, C(1) / te);
+    } else if constexpr (COLL == TLBM_COLL_BGK) {
       if (need_pi) {
         C feq[Q], fneq[Q], pi[6];
         equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
@@ -228,7 +264,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
     for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
   }
 
-  if constexpr (FEAT == FEAT_ALL) {
+  if constexpr (kAll) {
     if (a.force == nullptr) return;
     // one FP64 atomic per wave that holds solid cells
     if (__any(solid)) {
@@ -384,6 +420,14 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+namespace tlbm {
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace tlbm
+
+namespace {
+
 int check_launch(const char* what) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(TLBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
@@ -483,8 +527,11 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
       return fail(TLBM_ERR_INVALID, "unknown tau_mode %d", coll->tau_mode);
     if (coll->tau_mode == TLBM_TAU_FIELD && coll->tau_eff == nullptr)
       return fail(TLBM_ERR_INVALID, "TLBM_TAU_FIELD needs tau_eff");
-    if (coll->tau_mode == TLBM_TAU_MLP)
-      return fail(TLBM_ERR_UNSUPPORTED, "TLBM_TAU_MLP is served by the closure entry points");
+    if (coll->tau_mode == TLBM_TAU_MLP) {
+      if (coll->mlp == nullptr) return fail(TLBM_ERR_INVALID, "TLBM_TAU_MLP needs the packed closure (coll->mlp)");
+      if (lattice != TLBM_D3Q19 || coll->kind != TLBM_COLL_BGK)
+        return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
+    }
     if (coll->kind == TLBM_COLL_MRT) {
       // MRT rate vector; negative marks the shear moments (rate = omega)
       const double se = coll->rate[0], seps = coll->rate[1], sq = coll->rate[2], spi = coll->rate[3];
@@ -521,7 +568,8 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
 
 template <class L, int S, int COLL, bool STREAM, int FEAT>
 int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
+  const size_t lds = (FEAT & FEAT_MLP) ? sizeof(mlp::Packed) : 0;
+  hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), lds, s, a);
   return check_launch("k_lbm");
 }
 
@@ -560,6 +608,15 @@ int run_coll(const Call& c) {
     int rc = fill_args(a, c.lattice, c.fin, c.is, c.fout, c.os, c.nz, c.ny, c.nx, c.coll, c.solid, c.bc, c.force, &bx,
                        &by, &nb);
     if (rc != TLBM_OK) return rc;
+    if (c.coll != nullptr && c.coll->tau_mode == TLBM_TAU_MLP) {
+      if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK && S != ST_F64) {
+        if (!c.stream_) return fail(TLBM_ERR_UNSUPPORTED, "the learned closure runs in the fused step");
+        if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_MLP_ALL>(a, bx, by, nb, c.s);
+        return launch_lbm<L, S, COLL, true, FEAT_MLP>(a, bx, by, nb, c.s);
+      } else {
+        return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK with fp32/16-bit storage");
+      }
+    }
     if (c.stream_) {
       if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_ALL>(a, bx, by, nb, c.s);
       return launch_lbm<L, S, COLL, true, FEAT_NONE>(a, bx, by, nb, c.s);

@@ -220,6 +220,9 @@ class FusedSolver3D:
     ``"D3Q27"``), the collision family, deviation storage for 16-bit dtypes,
     an optional solid mask (full-way bounce-back) and a boundary
     (``"channel_x"`` with ``u_bc`` = inlet velocity, or ``"lid_y"``).
+    ``closure=EddyViscosityMLP3D`` relaxes D3Q19 BGK with the learned eddy
+    viscosity, evaluated inside the step (``C_s`` then sets the strain's time
+    scale: the Smagorinsky τ of that constant, or τ itself for ``C_s=0``).
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
@@ -227,7 +230,7 @@ class FusedSolver3D:
                  collision: str = "BGK", rates: Sequence[float] | None = None, deviation: bool = False,
                  C_s: float = 0.0, delta: float = 1.0, solid: torch.Tensor | None = None,
                  boundary: str | None = None, u_bc=(0.0, 0.0, 0.0), layout: str = "padded",
-                 **_tiles) -> None:
+                 closure=None, **_tiles) -> None:
         if precision is not None:
             policy = PrecisionPolicy.parse(precision)
             if policy.compute_dtype != torch.float32:
@@ -264,6 +267,14 @@ class FusedSolver3D:
         self._store = nat.store_code(dtype, self.deviation)
         r = tuple(rates) if rates is not None else _DEFAULT_RATES.get((lat, col), ())
         mode = nat.TAU_SMAG if C_s * C_s * delta * delta > 0.0 else nat.TAU_CONST
+        self.closure = closure
+        if closure is not None:
+            if lat != "D3Q19" or col != "BGK" or dtype == torch.float64:
+                raise NotImplementedError("the learned closure runs in the D3Q19 BGK kernel with fp32 or 16-bit "
+                                          "storage")
+            if not hasattr(closure, "packed"):
+                raise TypeError("closure must be an EddyViscosityMLP3D")
+            mode = nat.TAU_MLP
         self._coll = nat.make_collision(COLLISIONS[col], self.tau, tau_mode=mode,
                                         cs_delta_sq=C_s * C_s * delta * delta, rates=r)
         self.solid = None
@@ -299,6 +310,8 @@ class FusedSolver3D:
         out = self._buf_b if f.data_ptr() == self._buf_a.data_ptr() else self._buf_a
         if force is not None and self.solid is None:
             raise ValueError("force needs a solid mask")
+        if self.closure is not None:
+            self._coll.mlp = self.closure.packed.data_ptr()
         return _launch_step(f, out, self._coll, solid=self.solid, bc=self._bc, force=force,
                             deviation=self.deviation, what="FusedSolver3D.step")
 
