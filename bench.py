@@ -132,6 +132,45 @@ def measured_traffic(n: int):
     return None
 
 
+def cavities_form(torch, dev, steps=100):
+    """64 lid-driven cavities (edges U{32..160}, Re log-uniform [100, 1000]): one autotuned launch per step
+    replayed from a HIP graph, against 64 single-cavity fused launches per step."""
+    import tensorlbm_amd as tl
+    from tensorlbm_amd.cavities import BatchedCavities, cavity_batch_config, wall_mask
+
+    edges, taus = cavity_batch_config(64, seed=SEED)
+    b = BatchedCavities(edges, taus, u_lid=0.1, device=dev, dtype=torch.float32)
+    b.fill_equilibrium()
+    tune = b.autotune()
+    ms_graph = b.time_steps(steps, graph=True, graph_steps=20) * 1e3
+    ms_eager = b.time_steps(steps // 2, graph=False) * 1e3
+    cells = b.num_cells
+    ok = bool(torch.isfinite(b.populations(0)).all())
+    b.close()
+    del b
+    solvers = []
+    for e, t in zip(edges, taus):
+        s = tl.FusedSolver3D(e, e, e, t, dev, torch.float32, solid=wall_mask(e, dev), boundary="lid_y",
+                             u_bc=(0.1, 0.0, 0.0))
+        f = tl.equilibrium3d(torch.ones((e, e, e), device=dev), *(torch.zeros((e, e, e), device=dev),) * 3)
+        solvers.append([s, f])
+
+    def seq_step(_):
+        for item in solvers:
+            item[1] = item[0].step(item[1])
+        return None
+
+    ms_seq, _, _ = time_steps(seq_step, None, steps // 4, 3, torch)
+    del solvers
+    torch.cuda.empty_cache()
+    return {"cells": cells, "edges_min_max": [min(edges), max(edges)], "grid": tune and min(tune, key=tune.get),
+            "autotune_ms": {str(k): round(v * 1e3, 4) for k, v in tune.items()},
+            "ms_per_step_graph": round(ms_graph, 4), "mlups_graph": round(cells / (ms_graph * 1e-3) / 1e6, 1),
+            "ms_per_step_eager": round(ms_eager, 4), "ms_per_step_64_sequential": round(ms_seq, 4),
+            "mlups_64_sequential": round(cells / (ms_seq * 1e-3) / 1e6, 1),
+            "hbm_frac_graph": round(cells * 152 / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "finite": ok}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -154,13 +193,16 @@ def main():
     solid, f0 = channel_case(n, dev, torch)
     q = 19
 
-    def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True):
+    closure = None
+
+    def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True, mlp=False):
         f = f0.to(dtype)
         if deviation:
             f = (f0 - tl.lattice.D3Q19.W_EXACT64.to(dev).view(19, 1, 1, 1)).to(dtype)
         solver = tl.FusedSolver3D(n, n, n, tau=tau, device=dev, dtype=dtype, collision=collision,
                                   deviation=deviation, solid=solid if mask else None,
-                                  boundary="channel_x" if boundary else None, u_bc=(u_in, 0.0, 0.0))
+                                  boundary="channel_x" if boundary else None, u_bc=(u_in, 0.0, 0.0),
+                                  closure=closure if mlp else None, C_s=0.17 if mlp else 0.0)
         ms, wall, f = time_steps(solver.step, f, args.steps, args.warmup, torch)
         bpc = 2 * q * torch.tensor([], dtype=dtype).element_size()
         gbs = bpc * cells / (ms * 1e-3) / 1e9
@@ -178,6 +220,16 @@ def main():
         forms["bgk_fp16dev_channel_sphere"] = form(torch.float16, deviation=True)
         forms["bgk_bf16dev_channel_sphere"] = form(torch.bfloat16, deviation=True)
         forms["bgk_fp16dev_periodic_nomask"] = form(torch.float16, deviation=True, boundary=False, mask=False)
+        # learned closure evaluated in the collision (7 -> 32 -> 32 -> 1 on the matrix cores); the weights
+        # are seeded (throughput does not depend on their values), the inputs normalised to the strain scale
+        from tensorlbm_amd.closure import EddyViscosityMLP3D
+
+        closure = EddyViscosityMLP3D(dev, seed=SEED)
+        closure.set_feature_stats(torch.zeros(7), torch.full((7,), 1e-3))
+        forms["bgk_fp16dev_mlp_channel_sphere"] = form(torch.float16, deviation=True, mlp=True)
+        forms["bgk_fp32_mlp_channel_sphere"] = form(torch.float32, mlp=True)
+        forms["fp16dev_mlp_speedup_vs_fp32"] = round(forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"] / head["mlups"], 3)
+        forms["cavities_64"] = cavities_form(torch, dev)
     if args.copy_probe:
         src = f0.clone()
         dst = torch.empty_like(src)

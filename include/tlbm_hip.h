@@ -196,6 +196,46 @@ int tlbm_coarsen_moments(int lattice, int store, const void* f, const int64_t* s
 int tlbm_sgs_samples(const float* fields, int nz, int ny, int nx, float* feats, float* labels,
                      tlbm_stream_t stream);
 
+/*
+ * Batched lid-driven cavities (benchmark case of many independent cubic D3Q19 BGK cavities).
+ * Each cavity of edge L is the reference's cavity step chain on an L^3 grid: pull stream,
+ * Zou-He moving lid on y = L-1 (zou_he_moving_lid_3d, boundaries3d.py:324-404, every cell of
+ * the plane), full-way bounce-back walls x = 0, x = L-1, y = 0, z = 0, z = L-1
+ * (bounce_back_cells_3d, boundaries3d.py:112-132; walls of cases/cavity.py:72-79 closed in z),
+ * then BGK (collide_bgk3d, solver3d.py:100-104) -- per cavity the same result as tlbm_step
+ * with that wall mask and TLBM_BC_LID_Y.  No reference entry point runs a batch in one launch.
+ *
+ * One buffer holds all cavities: cavity i at element offset `offset`, layout [z][q][y][x]
+ * with the (y, x) plane padded to a multiple of 64 elements.  Up to TLBM_CAVITIES_MAX
+ * cavities; the table is passed by value to the kernel (no device table).
+ */
+#define TLBM_CAVITIES_MAX 128
+
+typedef struct tlbm_cavity {
+  int32_t edge;         /* L, 3..1024 */
+  int32_t reserved;
+  double tau;           /* BGK relaxation time (> 0.5) */
+  int64_t offset;       /* element offset in the batch buffer (tlbm_cavities_layout) */
+  int64_t tiles_before; /* 256-cell tiles of the preceding cavities (tlbm_cavities_layout) */
+} tlbm_cavity;
+
+/* host only: fill offset / tiles_before from the edges; totals in elements and tiles */
+int tlbm_cavities_layout(tlbm_cavity* cavs, int n, int64_t* total_elements, int64_t* total_tiles);
+
+/* one step of every cavity, f_in -> f_out (distinct buffers of `capacity` elements; store
+ * F32, F64, F16_DEV or BF16_DEV).  grid = 0: one workgroup per tile; grid > 0: a persistent
+ * grid of that many workgroups striding over the tiles. */
+int tlbm_cavities_step(int store, const void* f_in, void* f_out, int64_t capacity, const tlbm_cavity* cavs,
+                       int n, double u_lid, int grid, tlbm_stream_t stream);
+
+/* Capture `steps` (even) ping-pong steps a -> b -> a ... into a HIP graph; after a launch the
+ * state is back in f_a.  The graph keeps the buffer pointers: free it before the buffers. */
+typedef struct tlbm_graph tlbm_graph;
+int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const tlbm_cavity* cavs, int n,
+                        double u_lid, int grid, int steps, tlbm_graph** graph);
+int tlbm_graph_launch(tlbm_graph* graph, tlbm_stream_t stream);
+int tlbm_graph_destroy(tlbm_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

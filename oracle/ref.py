@@ -808,3 +808,65 @@ __all__ = [
     "cumulant_to_central", "sphere_mask", "obstacle_force", "channel_bc",
     "zou_he_moving_lid", "correct_mass", "fused_obstacle_step",
 ]
+
+
+# ---------------------------------------------------------------------------
+# lid-driven cavity step (kernel semantics)
+# ---------------------------------------------------------------------------
+
+
+def cavity_walls(edge: int) -> np.ndarray:
+    """Five stationary walls of a cubic cavity: x = 0, x = L−1, y = 0, z = 0, z = L−1
+    (cases/cavity.py:72-79 walls, closed in z)."""
+    m = np.zeros((edge, edge, edge), dtype=bool)
+    m[:, :, 0] = m[:, :, -1] = True
+    m[:, 0, :] = True
+    m[0] = m[-1] = True
+    return m
+
+
+def cavity_step(f: np.ndarray, tau: float, u_lid: float) -> np.ndarray:
+    """Pull stream, Zou-He lid on y = L−1 (every cell of the plane, boundaries3d.py:324-404),
+    full-way bounce-back on the walls, BGK with the fused kernels' floored-density equilibrium."""
+    fin = zou_he_moving_lid(stream(f), u_lid)
+    fin = bounce_back(fin, cavity_walls(f.shape[1]))
+    feq, _, _ = _feq_safe(fin)
+    return fin - (1.0 / tau) * (fin - feq)
+
+
+# ---------------------------------------------------------------------------
+# D2Q9 central-moment ("cumulant") collision
+# ---------------------------------------------------------------------------
+
+
+def collide_cumulant_d2q9(f, tau, omega_b=1.0, omega_3=1.0, omega_4=1.0):
+    """cumulant.py:116-311: raw moments m_pq (p, q ≤ 2) → central moments κ_pq by the binomial
+    shift with (u_x, u_y) → relax (deviatoric stress at 1/τ, trace at ω_b, κ21/κ12 at ω_3, κ22 at
+    ω_4) towards the central moments of the discrete equilibrium → shift back → populations."""
+    lat = D2Q9
+    rho, ux, uy, _ = macroscopic(f)
+    feq = equilibrium(lat, rho, ux, uy)
+    cx = lat.c[:, 0].astype(float).reshape(9, 1, 1)
+    cy = lat.c[:, 1].astype(float).reshape(9, 1, 1)
+    orders = [(2, 0), (0, 2), (1, 1), (2, 1), (1, 2), (2, 2)]
+
+    def central(g):
+        return {pq: (g * (cx - ux) ** pq[0] * (cy - uy) ** pq[1]).sum(0) for pq in orders}
+
+    k, ke = central(f), central(feq)
+    om = 1.0 / tau
+    ks = {pq: k[pq] - om * (k[pq] - ke[pq]) for pq in ((2, 0), (0, 2), (1, 1))}
+    t, te = k[(2, 0)] + k[(0, 2)], ke[(2, 0)] + ke[(0, 2)]
+    d = 0.5 * ((t - omega_b * (t - te)) - (ks[(2, 0)] + ks[(0, 2)]))
+    ks[(2, 0)] = ks[(2, 0)] + d
+    ks[(0, 2)] = ks[(0, 2)] + d
+    for pq, w in (((2, 1), omega_3), ((1, 2), omega_3), ((2, 2), omega_4)):
+        ks[pq] = k[pq] - w * (k[pq] - ke[pq])
+    # populations from the 9 central moments: solve the 9x9 system per cell
+    # (rows: κ00 = ρ, κ10 = κ01 = 0 and the six relaxed moments)
+    cols = [np.ones_like(cx * ux), cx - ux, cy - uy] + [(cx - ux) ** p * (cy - uy) ** q for p, q in orders]
+    A = np.stack([np.broadcast_to(c, f.shape) for c in cols])  # (9 moments, 9 populations, ny, nx)
+    b = np.stack([rho, np.zeros_like(rho), np.zeros_like(rho)] + [ks[pq] for pq in orders])
+    A2 = np.moveaxis(A, (0, 1), (-2, -1))
+    b2 = np.moveaxis(b, 0, -1)[..., None]
+    return np.moveaxis(np.linalg.solve(A2, b2)[..., 0], -1, 0)

@@ -47,6 +47,19 @@ class Collision(ctypes.Structure):
     ]
 
 
+class Cavity(ctypes.Structure):
+    _fields_ = [
+        ("edge", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("tau", ctypes.c_double),
+        ("offset", ctypes.c_int64),
+        ("tiles_before", ctypes.c_int64),
+    ]
+
+
+CAVITIES_MAX = 128
+
+
 class Boundary(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
@@ -81,6 +94,11 @@ _SIGNATURES = {
                        ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
     "tlbm_coarsen_moments": ([_I, _I, _VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "tlbm_sgs_samples": ([_VP, _I, _I, _I, _VP, _VP, _VP], _I),
+    "tlbm_cavities_layout": ([_VP, _I, _VP, _VP], _I),
+    "tlbm_cavities_step": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _VP], _I),
+    "tlbm_cavities_graph": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _I, _VP], _I),
+    "tlbm_graph_launch": ([_VP, _VP], _I),
+    "tlbm_graph_destroy": ([_VP], _I),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -197,6 +197,14 @@ __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C*
 template <class L>
 struct CmTab;
 template <>
+struct CmTab<D2Q9> {
+  static constexpr const double (&M)[9][9] = tables::CM9;
+  static constexpr const double (&Mi)[9][9] = tables::CM9_INV;
+  static constexpr const int (&SX)[3][3] = tables::CM9_SHIFT_X;
+  static constexpr const int (&SY)[3][3] = tables::CM9_SHIFT_Y;
+  static constexpr int NG = 3;
+};
+template <>
 struct CmTab<D3Q19> {
   static constexpr const double (&M)[19][19] = tables::CM19;
   static constexpr const double (&Mi)[19][19] = tables::CM19_INV;
@@ -254,18 +262,22 @@ template <class L, typename C>
 __device__ __forceinline__ void to_central(C (&m)[L::Q], C ux, C uy, C uz) {
   shift1d<CmTab<L>::NG>(m, CmTab<L>::SX, ux, C(-1));
   shift1d<CmTab<L>::NG>(m, CmTab<L>::SY, uy, C(-1));
-  shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(-1));
+  if constexpr (L::Q != 9) shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(-1));
 }
 
 template <class L, typename C>
 __device__ __forceinline__ void to_raw(C (&m)[L::Q], C ux, C uy, C uz) {
-  shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(1));
+  if constexpr (L::Q != 9) shift1d<CmTab<L>::NG>(m, CmTab<L>::SZ, uz, C(1));
   shift1d<CmTab<L>::NG>(m, CmTab<L>::SY, uy, C(1));
   shift1d<CmTab<L>::NG>(m, CmTab<L>::SX, ux, C(1));
 }
 
 // Cascaded central-moment collision on f_neq (cascaded_collision.py:445-509).
-// rate = {s_bulk, s_3, s_4, s_5, s_6}.
+// rate = {s_bulk, s_3, s_4, s_5, s_6}.  On D2Q9 this is collide_cumulant_d2q9
+// (cumulant.py:116-311: trace at omega_b, deviatoric stress at omega, kappa21 /
+// kappa12 at omega_3, kappa22 at omega_4), rate = {omega_b, omega_3, omega_4}.
+// Relaxing the central moments of f_neq = f - f_eq at rate s is relaxing those
+// of f towards the central moments of f_eq, since the shift is linear in f.
 template <class L, typename C>
 __device__ __forceinline__ void cascaded(C (&f)[L::Q], const C (&feq)[L::Q], const Macro<C>& mc,
                                          C omega, const C* rate) {
@@ -275,6 +287,22 @@ __device__ __forceinline__ void cascaded(C (&f)[L::Q], const C (&feq)[L::Q], con
   for (int q = 0; q < Q; ++q) v[q] = f[q] - feq[q];
   moments<L>(v, m);
   to_central<L>(m, mc.ux, mc.uy, mc.uz);
+  if constexpr (Q == 9) {
+    // m: 00 10 01 20 02 11 21 12 22
+    const C tr = m[3] + m[4];
+    const C trs = (C(1) - rate[0]) * tr;
+    m[3] = (C(1) - omega) * (m[3] - C(0.5) * tr) + C(0.5) * trs;
+    m[4] = (C(1) - omega) * (m[4] - C(0.5) * tr) + C(0.5) * trs;
+    m[5] = (C(1) - omega) * m[5];
+    m[6] = (C(1) - rate[1]) * m[6];
+    m[7] = (C(1) - rate[1]) * m[7];
+    m[8] = (C(1) - rate[2]) * m[8];
+    to_raw<L>(m, mc.ux, mc.uy, mc.uz);
+    populations<L>(m, v);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = feq[q] + v[q];
+    return;
+  }
   const C tr = m[4] + m[5] + m[6];
   const C trs = (C(1) - rate[0]) * tr;
   const C third = C(1.0 / 3.0);

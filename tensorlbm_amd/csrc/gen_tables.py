@@ -6,7 +6,8 @@ Run ``python gen_tables.py`` in this directory; the output is committed.
   polynomial rows (solver3d.py:24-58), D3Q27 Gram-Schmidt rows
   (d3q27.py:545-633, orthogonalised here in exact rationals).
 * Raw-monomial moment matrices and 1-D binomial shift groups for the
-  cascaded / cumulant operators (cascaded_collision.py:60-187).
+  cascaded / cumulant operators (cascaded_collision.py:60-187; the D2Q9
+  moment set m00..m22 of cumulant.py:160-175).
 * Geier cumulant <-> central-moment transforms for D3Q27, derived from the
   set-partition (moment-cumulant) formula rather than typed in
   (the reference's coefficients are cumulant.py:674-815).
@@ -34,6 +35,7 @@ D3Q27_C = [
     (-1, 0, -1), (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1), (1, 1, 1), (-1, 1, 1),
     (1, -1, 1), (-1, -1, 1), (1, 1, -1), (-1, 1, -1), (1, -1, -1), (-1, -1, -1),
 ]
+DEG9 = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (2, 0, 0), (0, 2, 0), (1, 1, 0), (2, 1, 0), (1, 2, 0), (2, 2, 0)]
 DEG19 = [
     (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0),
     (1, 0, 1), (0, 1, 1), (2, 1, 0), (2, 0, 1), (1, 2, 0), (0, 2, 1), (1, 0, 2), (0, 1, 2),
@@ -191,8 +193,9 @@ def emit_matrix(name, m):
 def emit_groups(name, groups):
     lines = []
     for ax, g in zip("XYZ", groups):
-        flat = ", ".join(f"{{{a}, {b}, {c}}}" for a, b, c in g)
-        lines.append(f"static constexpr int {name}_{ax}[{len(g)}][3] = {{{flat}}};")
+        if g:  # a 2-D lattice has no z groups
+            flat = ", ".join(f"{{{a}, {b}, {c}}}" for a, b, c in g)
+            lines.append(f"static constexpr int {name}_{ax}[{len(g)}][3] = {{{flat}}};")
         lines.append(f"static constexpr int {name}_N{ax} = {len(g)};")
     return "\n".join(lines) + "\n"
 
@@ -225,7 +228,7 @@ def main():
     for name, m in (("MRT9", mrt_d2q9()), ("MRT19", mrt_d3q19()), ("MRT27", mrt_d3q27())):
         lines.append(emit_matrix(name, m))
         lines.append(emit_matrix(name + "_INV", inverse(m)))
-    for name, c, deg in (("CM19", D3Q19_C, DEG19), ("CM27", D3Q27_C, DEG27)):
+    for name, c, deg in (("CM9", D2Q9_C, DEG9), ("CM19", D3Q19_C, DEG19), ("CM27", D3Q27_C, DEG27)):
         m = monomial(c, deg)
         lines.append(emit_matrix(name, m))
         lines.append(emit_matrix(name + "_INV", inverse(m)))

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
       if constexpr (COLL == TLBM_COLL_MRT) {
         mrt<L>(f, feq, a.s, omega);
       } else if constexpr (COLL == TLBM_COLL_CM) {
-        if constexpr (Q != 9) cascaded<L>(f, feq, mc, omega, a.rate);
+        cascaded<L>(f, feq, mc, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
         cumulant19<L>(f, feq, pi, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_GEIER) {
@@ -576,7 +576,7 @@ int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb
 template <class L, int COLL>
 constexpr bool coll_built() {
   if (COLL == TLBM_COLL_NONE || COLL == TLBM_COLL_BGK || COLL == TLBM_COLL_MRT) return true;
-  if (COLL == TLBM_COLL_CM) return L::Q != 9;
+  if (COLL == TLBM_COLL_CM) return true;
   if (COLL == TLBM_COLL_CUMULANT) return L::Q == 19;
   if (COLL == TLBM_COLL_GEIER) return L::Q == 27;
   return false;
@@ -609,12 +609,12 @@ int run_coll(const Call& c) {
                        &by, &nb);
     if (rc != TLBM_OK) return rc;
     if (c.coll != nullptr && c.coll->tau_mode == TLBM_TAU_MLP) {
-      if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK && S != ST_F64) {
+      if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK) {
         if (!c.stream_) return fail(TLBM_ERR_UNSUPPORTED, "the learned closure runs in the fused step");
         if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_MLP_ALL>(a, bx, by, nb, c.s);
         return launch_lbm<L, S, COLL, true, FEAT_MLP>(a, bx, by, nb, c.s);
       } else {
-        return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK with fp32/16-bit storage");
+        return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
       }
     }
     if (c.stream_) {

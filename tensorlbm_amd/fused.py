@@ -269,9 +269,8 @@ class FusedSolver3D:
         mode = nat.TAU_SMAG if C_s * C_s * delta * delta > 0.0 else nat.TAU_CONST
         self.closure = closure
         if closure is not None:
-            if lat != "D3Q19" or col != "BGK" or dtype == torch.float64:
-                raise NotImplementedError("the learned closure runs in the D3Q19 BGK kernel with fp32 or 16-bit "
-                                          "storage")
+            if lat != "D3Q19" or col != "BGK":
+                raise NotImplementedError("the learned closure runs in the D3Q19 BGK kernel")
             if not hasattr(closure, "packed"):
                 raise TypeError("closure must be an EddyViscosityMLP3D")
             mode = nat.TAU_MLP

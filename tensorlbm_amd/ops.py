@@ -32,7 +32,7 @@ __all__ = [
     "equilibrium", "macroscopic", "equilibrium3d", "equilibrium3d_low_memory", "macroscopic3d",
     "macroscopic3d_low_memory", "equilibrium27", "macroscopic27", "collide_bgk", "collide_bgk3d",
     "collide_bgk27", "collide_mrt", "collide_mrt3d", "collide_mrt27", "collide_cascaded_d3q19",
-    "collide_cascaded_d3q27", "collide_cumulant_d3q19", "collide_cumulant_geier_d3q27",
+    "collide_cascaded_d3q27", "collide_cumulant_d2q9", "collide_cumulant_d3q19", "collide_cumulant_geier_d3q27",
     "collide_smagorinsky_bgk", "collide_smagorinsky_bgk3d", "collide_smagorinsky_bgk27", "stream",
     "stream3d", "stream3d_roll", "stream27", "stream27_roll", "correct_mass", "correct_mass3d",
     "correct_mass27", "bounce_back_cells", "bounce_back_cells_3d", "bounce_back_cells_27",
@@ -216,6 +216,19 @@ def collide_mrt27(f, tau, s_e=1.19, s_eps=1.4, s_q=1.2, s_pi=None):
     coll = nat.make_collision(nat.COLL_MRT, _tau(tau, "collide_mrt27"), eq_rho_floor=False,
                               rates=(s_e, s_eps, s_q, s_pi))
     return _collide(f, 27, coll, "collide_mrt27")
+
+
+def collide_cumulant_d2q9(f, tau, omega_b=1.0, omega_3=1.0, omega_4=1.0):
+    """D2Q9 central-moment ("cumulant") collision (cumulant.py:116-311).
+
+    Deviatoric stress at ω = 1/τ, trace at ``omega_b``, κ21/κ12 at ``omega_3``, κ22 at ``omega_4``;
+    equilibrium central moments from the discrete equilibrium, so f_eq is a fixed point.
+    """
+    if f.ndim != 3 or f.shape[0] != 9:
+        raise ValueError(f"D2Q9 populations must have shape (9, ny, nx), got {tuple(f.shape)}")
+    tau = _tau(tau, "collide_cumulant_d2q9")
+    coll = nat.make_collision(nat.COLL_CM, tau, eq_rho_floor=False, rates=(omega_b, omega_3, omega_4))
+    return _collide(f, 9, coll, "collide_cumulant_d2q9")
 
 
 def collide_cascaded_d3q19(f, tau, s_bulk=1.0, s_3=1.0, s_4=1.0):

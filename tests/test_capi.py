@@ -77,3 +77,41 @@ def test_store_codes():
         nat.store_code(torch.int32)
     with pytest.raises(ValueError):
         nat.store_code(torch.float32, deviation=True)
+
+
+def test_cavity_layout_is_host_only_and_validates():
+    lib = nat.lib()
+    tab = (nat.Cavity * 3)()
+    for i, e in enumerate((8, 33, 5)):
+        tab[i].edge = e
+        tab[i].tau = 0.6
+    tot, tiles = ctypes.c_int64(), ctypes.c_int64()
+    assert lib.tlbm_cavities_layout(tab, 3, ctypes.byref(tot), ctypes.byref(tiles)) == nat.OK
+    planes = [(e * e + 63) // 64 * 64 for e in (8, 33, 5)]
+    assert [tab[i].offset for i in range(3)] == [0, 8 * 19 * planes[0], 8 * 19 * planes[0] + 33 * 19 * planes[1]]
+    assert tot.value == sum(e * 19 * p for e, p in zip((8, 33, 5), planes))
+    assert [tab[i].tiles_before for i in range(3)] == [0, 8, 8 + 33 * 5]
+    assert tiles.value == 8 + 33 * 5 + 5
+    p, p2 = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x100000)
+    step = lib.tlbm_cavities_step
+    assert step(0, p, p, tot.value, tab, 3, 0.1, 0, None) == nat.ERR_ALIAS
+    assert step(0, p, p2, tot.value - 1, tab, 3, 0.1, 0, None) == nat.ERR_SHAPE
+    assert step(2, p, p2, tot.value, tab, 3, 0.1, 0, None) == nat.ERR_DTYPE
+    assert step(0, p, p2, tot.value, tab, 3, 0.1, -1, None) == nat.ERR_INVALID
+    tab[1].tau = 0.5
+    assert step(0, p, p2, tot.value, tab, 3, 0.1, 0, None) == nat.ERR_INVALID
+    tab[1].tau = 0.6
+    tab[2].tiles_before += 1
+    assert step(0, p, p2, tot.value, tab, 3, 0.1, 0, None) == nat.ERR_INVALID
+    bad = (nat.Cavity * 1)()
+    bad[0].edge = 2
+    assert lib.tlbm_cavities_layout(bad, 1, None, None) == nat.ERR_SHAPE
+    h = ctypes.c_void_p()
+    assert lib.tlbm_cavities_graph(0, p, p2, tot.value, tab, 3, 0.1, 0, 3, ctypes.byref(h)) == nat.ERR_INVALID
+    assert lib.tlbm_graph_destroy(None) == nat.OK
+
+
+def test_mlp_sizes_without_a_device():
+    v = [ctypes.c_int64() for _ in range(4)]
+    assert nat.lib().tlbm_mlp_sizes(*[ctypes.byref(x) for x in v]) == nat.OK
+    assert [x.value for x in v] == [1345, 1359, 4544, 64]

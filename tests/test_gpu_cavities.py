@@ -1,0 +1,141 @@
+"""Batched lid-driven cavities: one launch (or one graph) for many cavities.
+
+Per cavity the batch must equal the single-grid fused step with the wall mask and the
+moving-lid boundary (bit for bit in fp32) and the oracle's cavity chain (fp64).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import tensorlbm_amd as tl
+from oracle import ref
+from tensorlbm_amd.cavities import BatchedCavities, cavity_batch_config, wall_mask
+from tests import _fields as F
+
+pytestmark = pytest.mark.gpu
+
+EDGES = [8, 11, 13, 16, 5]
+TAUS = [0.8, 0.55, 0.62, 0.51, 1.3]
+U_LID = 0.1
+
+
+def _batch(dev, dtype=torch.float64, deviation=False, seed=0):
+    b = BatchedCavities(EDGES, TAUS, u_lid=U_LID, device=dev, dtype=dtype, deviation=deviation)
+    states = []
+    for i, e in enumerate(EDGES):
+        f0 = F.random_state(19, (e, e, e), seed=seed + i, noise=5e-3, u_amp=0.03)
+        states.append(f0)
+        w = ref.D3Q19.w.reshape(19, 1, 1, 1) if deviation else 0.0
+        b.cavity(i).copy_(torch.tensor(f0 - w, device=dev).to(dtype))
+    return b, states
+
+
+def test_layout_offsets_are_aligned_and_disjoint(dev):
+    b, _ = _batch(dev)
+    end = 0
+    for i, e in enumerate(EDGES):
+        off = b.offset(i)
+        assert off % 64 == 0 and off >= end
+        end = off + e * 19 * ((e * e + 63) // 64 * 64)
+    assert end == b.total_elements
+    assert b.total_tiles == sum(e * ((e * e + 255) // 256) for e in EDGES)
+
+
+def test_batch_matches_oracle_fp64(dev):
+    b, states = _batch(dev)
+    for _ in range(3):
+        b.step()
+        states = [ref.cavity_step(s, t, U_LID) for s, t in zip(states, TAUS)]
+    for i in range(len(EDGES)):
+        np.testing.assert_allclose(b.populations(i).cpu().numpy(), states[i], rtol=1e-12, atol=1e-14,
+                                   err_msg=f"cavity {i}")
+
+
+def test_batch_equals_single_grid_fused_step_bitwise(dev):
+    b, states = _batch(dev, torch.float32)
+    singles = []
+    for i, e in enumerate(EDGES):
+        s = tl.FusedSolver3D(e, e, e, TAUS[i], dev, torch.float32, solid=wall_mask(e, dev), boundary="lid_y",
+                             u_bc=(U_LID, 0.0, 0.0), layout="contiguous")
+        singles.append((s, torch.tensor(states[i], device=dev, dtype=torch.float32)))
+    for _ in range(4):
+        b.step()
+        singles = [(s, s.step(f)) for s, f in singles]
+    for i, (_, f) in enumerate(singles):
+        assert torch.equal(b.populations(i), f), f"cavity {i}"
+
+
+@pytest.mark.parametrize("grid", [1, 7, 64, 4096])
+def test_persistent_grids_match_one_workgroup_per_tile(dev, grid):
+    b0, _ = _batch(dev, torch.float32, seed=3)
+    b1, _ = _batch(dev, torch.float32, seed=3)
+    b1.grid = grid
+    for _ in range(3):
+        b0.step()
+        b1.step()
+    assert torch.equal(b0.a, b1.a)
+
+
+def test_graph_replays_match_eager_steps(dev):
+    """Each replay of the captured ping-pong graph equals the same number of eager launches,
+    with eager work and synchronisation between the replays."""
+    be, _ = _batch(dev, torch.float32, seed=5)
+    bg, _ = _batch(dev, torch.float32, seed=5)
+    for rep in range(3):
+        bg.run(6, graph=True, graph_steps=6)
+        be.run(6, graph=False)
+        torch.cuda.synchronize()
+        assert torch.equal(bg.a, be.a), f"replay {rep}"
+        _ = (bg.a * 2.0).sum().item()  # unrelated eager work between replays
+    bg.run(9, graph=True, graph_steps=4)  # two graph launches + one eager step
+    be.run(9, graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(bg.a, be.a)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_deviation_storage_tracks_fp32(dev, dtype):
+    b32, _ = _batch(dev, torch.float32, seed=7)
+    b16, _ = _batch(dev, dtype, deviation=True, seed=7)
+    for _ in range(20):
+        b32.step()
+        b16.step()
+    for i in range(len(EDGES)):
+        a, c = b16.populations(i).double(), b32.populations(i).double()
+        assert torch.isfinite(a).all()
+        # 16-bit rounding of f - w (|f - w| < 0.02 here) per step, 20 steps
+        tol = 5e-4 if dtype == torch.float16 else 4e-3
+        assert float((a - c).abs().max()) < tol, (i, float((a - c).abs().max()))
+
+
+def test_lid_drives_the_flow(dev):
+    edges, taus = [24, 17], [0.56, 0.6]
+    b = BatchedCavities(edges, taus, u_lid=U_LID, device=dev, dtype=torch.float32)
+    b.fill_equilibrium()
+    b.run(300, graph=True, graph_steps=20)
+    for i, e in enumerate(edges):
+        rho, ux, uy, uz = tl.macroscopic3d(b.populations(i))
+        inner = (slice(1, e - 1), slice(1, e - 1), slice(1, e - 1))
+        top = ux[1:e - 1, e - 2, 1:e - 1].mean()
+        assert float(top) > 0.2 * U_LID  # the layer under the lid is dragged along +x
+        assert abs(float(rho[inner].mean()) - 1.0) < 1e-2
+        assert float(uy[inner].abs().max()) > 1e-3  # a recirculating vortex, not a plain shear
+
+
+def test_config_generator_matches_its_formula():
+    edges, taus = cavity_batch_config(64, seed=0)
+    assert len(edges) == 64 and min(edges) >= 32 and max(edges) <= 160
+    re = 3 * 0.1 * np.array(edges) / (np.array(taus) - 0.5)
+    assert re.min() >= 100 - 1e-9 and re.max() <= 1000 + 1e-9
+
+
+def test_argument_errors(dev):
+    with pytest.raises(ValueError):
+        BatchedCavities([8, 2], [0.6, 0.6], device=dev)
+    with pytest.raises(ValueError):
+        BatchedCavities([8], [0.5], device=dev).step()
+    with pytest.raises(ValueError):
+        BatchedCavities([8] * 129, [0.6] * 129, device=dev)

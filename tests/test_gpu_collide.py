@@ -97,6 +97,13 @@ def test_mrt_d2q9_tau_field(dev):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cumulant_d2q9(dtype, dev):
+    f = _state(9, noise=3e-2)
+    got = tl.collide_cumulant_d2q9(_gpu(f, dev, dtype), 0.61, omega_b=1.3, omega_3=0.7, omega_4=1.6)
+    _cmp(got, ref.collide_cumulant_d2q9(f, 0.61, 1.3, 0.7, 1.6), dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_cascaded_d3q19(dtype, dev):
     f = _state(19)
     got = tl.collide_cascaded_d3q19(_gpu(f, dev, dtype), 0.57, s_bulk=1.3, s_3=1.1, s_4=0.9)

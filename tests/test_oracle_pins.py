@@ -465,3 +465,64 @@ def test_zou_he_lid_imposes_velocity():
 def test_correct_mass():
     f = F.random_state(19, (3, 4, 5), seed=19)
     assert abs(ref.correct_mass(f, 42.0).sum() - 42.0) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# D2Q9 central-moment ("cumulant") collision
+# ---------------------------------------------------------------------------
+
+
+def test_cumulant_d2q9_reduces_to_bgk_and_equilibrium():
+    f = F.random_state(9, (1, 6, 7), seed=3, noise=2e-2)[:, 0]
+    tau = 0.62
+    np.testing.assert_allclose(ref.collide_cumulant_d2q9(f, tau, 1 / tau, 1 / tau, 1 / tau), ref.collide_bgk(f, tau),
+                               rtol=0, atol=1e-15)
+    rho, ux, uy, _ = ref.macroscopic(f)
+    np.testing.assert_allclose(ref.collide_cumulant_d2q9(f, 1.0), ref.equilibrium(D2Q9, rho, ux, uy), atol=1e-15)
+    out = ref.collide_cumulant_d2q9(f, tau, 1.3, 0.7, 1.6)
+    r2, ux2, uy2, _ = ref.macroscopic(out)
+    np.testing.assert_allclose(r2, rho, atol=1e-15)
+    np.testing.assert_allclose(ux2, ux, atol=1e-15)
+    np.testing.assert_allclose(uy2, uy, atol=1e-15)
+    feq = ref.equilibrium(D2Q9, rho, ux, uy)
+    np.testing.assert_allclose(ref.collide_cumulant_d2q9(feq, tau, 1.3, 0.7, 1.6), feq, atol=1e-15)
+
+
+def test_cumulant_d2q9_shift_formulas_of_the_reference():
+    """The reference's explicit forward shifts (cumulant.py:177-198) equal brute-force central moments."""
+    f = F.random_state(9, (1, 4, 5), seed=4, noise=3e-2)[:, 0]
+    rho, ux, uy, _ = ref.macroscopic(f)
+    cx = D2Q9.c[:, 0].reshape(9, 1, 1).astype(float)
+    cy = D2Q9.c[:, 1].reshape(9, 1, 1).astype(float)
+    m = {(p, q): (f * cx**p * cy**q).sum(0) for p in range(3) for q in range(3)}
+    k = {(p, q): (f * (cx - ux) ** p * (cy - uy) ** q).sum(0) for p in range(3) for q in range(3)}
+    np.testing.assert_allclose(k[(2, 1)], m[(2, 1)] - uy * m[(2, 0)] - 2 * ux * m[(1, 1)] + 2 * ux**2 * uy * rho,
+                               atol=1e-15)
+    np.testing.assert_allclose(k[(1, 2)], m[(1, 2)] - ux * m[(0, 2)] - 2 * uy * m[(1, 1)] + 2 * ux * uy**2 * rho,
+                               atol=1e-15)
+    k22 = (m[(2, 2)] - 2 * ux * m[(1, 2)] - 2 * uy * m[(2, 1)] + ux**2 * m[(0, 2)] + 4 * ux * uy * m[(1, 1)]
+           + uy**2 * m[(2, 0)] - 3 * ux**2 * uy**2 * rho)
+    np.testing.assert_allclose(k[(2, 2)], k22, atol=1e-15)
+
+
+def test_cumulant_d2q9_relaxes_each_mode_at_its_rate():
+    """Post-collision central moments, measured by brute force, sit at (1 − s) of their
+    pre-collision distance from the equilibrium's; the trace uses omega_b."""
+    f = F.random_state(9, (1, 4, 6), seed=5, noise=3e-2)[:, 0]
+    tau, ob, o3, o4 = 0.7, 1.25, 0.6, 1.7
+    out = ref.collide_cumulant_d2q9(f, tau, ob, o3, o4)
+    rho, ux, uy, _ = ref.macroscopic(f)
+    feq = ref.equilibrium(D2Q9, rho, ux, uy)
+    cx = D2Q9.c[:, 0].reshape(9, 1, 1).astype(float)
+    cy = D2Q9.c[:, 1].reshape(9, 1, 1).astype(float)
+
+    def kap(g, p, q):
+        return (g * (cx - ux) ** p * (cy - uy) ** q).sum(0)
+
+    for (p, q), s in (((1, 1), 1 / tau), ((2, 1), o3), ((1, 2), o3), ((2, 2), o4)):
+        np.testing.assert_allclose(kap(out, p, q) - kap(feq, p, q), (1 - s) * (kap(f, p, q) - kap(feq, p, q)),
+                                   atol=1e-15)
+    tr = lambda g: kap(g, 2, 0) + kap(g, 0, 2)  # noqa: E731
+    dv = lambda g: kap(g, 2, 0) - kap(g, 0, 2)  # noqa: E731
+    np.testing.assert_allclose(tr(out) - tr(feq), (1 - ob) * (tr(f) - tr(feq)), atol=1e-15)
+    np.testing.assert_allclose(dv(out) - dv(feq), (1 - 1 / tau) * (dv(f) - dv(feq)), atol=1e-15)
