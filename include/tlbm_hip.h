@@ -52,8 +52,10 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
 #define TLBM_COLL_NONE 0     /* stream only */
 #define TLBM_COLL_BGK 1      /* solver.py:67-71, solver3d.py:100-104, d3q27.py:423-435 */
 #define TLBM_COLL_MRT 2      /* solver.py:249-335, solver3d.py:107-160, d3q27.py:643-726 */
-#define TLBM_COLL_CM 3       /* cascaded central moments, cascaded_collision.py:445-509 */
-#define TLBM_COLL_CUMULANT 4 /* D3Q19 Hermite-regularised, cumulant.py:1159-1377 */
+#define TLBM_COLL_CM 3       /* cascaded central moments, cascaded_collision.py:445-509;
+                                D2Q9: collide_cumulant_d2q9, cumulant.py:116-311 */
+#define TLBM_COLL_CUMULANT 4 /* D3Q19 Hermite-regularised, cumulant.py:1159-1377;
+                                D3Q27 legacy operator on f_neq, cumulant.py:935-1047 */
 #define TLBM_COLL_GEIER 5    /* D3Q27 Geier 2015 cumulants, cumulant.py:1050-1156 */
 
 /* source of the shear relaxation rate */
@@ -69,6 +71,17 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
 #define TLBM_BC_CHANNEL_X 1 /* equilibrium inlet x=0 (rho from x=1), copy outlet x=nx-1,
                                boundaries3d.py:407-443 */
 #define TLBM_BC_LID_Y 2     /* Zou-He moving lid on y=ny-1, boundaries3d.py:324-404 */
+#define TLBM_BC_FAR_FIELD 3 /* free-stream equilibrium (rho = 1, u) on the inlet x=0 and the lateral
+                               faces named in `faces`, copy outlet x=nx-1 (far_field_bc_3d,
+                               boundaries3d.py:446-537; lateral faces win on shared edges) */
+
+/* tlbm_boundary.faces bits for TLBM_BC_FAR_FIELD */
+#define TLBM_FACE_YM 1           /* y = 0 plane */
+#define TLBM_FACE_YP 2           /* y = ny-1 */
+#define TLBM_FACE_ZM 4           /* z = 0 */
+#define TLBM_FACE_ZP 8           /* z = nz-1 */
+#define TLBM_FACE_XM_PERIODIC 16 /* leave the inlet plane to the periodic stream */
+#define TLBM_FACE_XP_PERIODIC 32 /* leave the outlet plane to the periodic stream */
 
 typedef struct tlbm_collision {
   int32_t kind;         /* TLBM_COLL_* */
@@ -87,8 +100,8 @@ typedef struct tlbm_collision {
 
 typedef struct tlbm_boundary {
   int32_t kind;   /* TLBM_BC_* */
-  int32_t reserved;
-  double u[3];    /* inlet / lid velocity */
+  int32_t faces;  /* TLBM_BC_FAR_FIELD: TLBM_FACE_* bits; 0 otherwise */
+  double u[3];    /* inlet / lid / free-stream velocity */
 } tlbm_boundary;
 
 /* library */
@@ -150,6 +163,13 @@ int tlbm_moment_sums(int lattice, int store, const void* f, const int64_t* strid
 /* ADD the Ladd force 2 sum_solid sum_q c_q f to out[0..3] (obstacles.py:91-130), FP64. */
 int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* strides, int nz,
                         int ny, int nx, const int8_t* solid, double* out, tlbm_stream_t stream);
+
+/* ADD sum over cells of the clamped Smagorinsky tau_eff of each cell's non-equilibrium stress
+ * (turbulence.py:183-208 with |Pi| of :143-180 / :436-469) to out[0] (device double, FP64).
+ * collide_cumulant_d3q27's C_s > 0 path relaxes with the domain mean (cumulant.py:986-997). */
+int tlbm_smagorinsky_tau_sum(int lattice, int store, const void* f, const int64_t* strides, int nz,
+                             int ny, int nx, double tau, double cs_delta_sq, double* out,
+                             tlbm_stream_t stream);
 
 /* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
 int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx,

@@ -351,7 +351,18 @@ def mrt_rates(lat: Lattice, tau: float, s_e: float, s_eps: float, s_q: float, s_
     return [0.0] * 4 + [s_e] + [s_nu] * 5 + [s_q] * 9 + [s_eps] + [s_pi] * 7
 
 
-def collide_mrt(f: np.ndarray, tau: float, s_e=None, s_eps=None, s_q=None, s_pi=None) -> np.ndarray:
+def collide_smagorinsky_mrt(f: np.ndarray, tau: float, C_s: float = 0.1, s_e=None, s_eps=None, s_q=None,
+                            s_pi=None) -> np.ndarray:
+    """MRT with the stress rows relaxed at the per-cell Smagorinsky 1/τ_eff
+    (turbulence.py:270-342 D2Q9, :345-433 D3Q19): τ_eff from |Π^neq| of f − f_eq(ρ, u)."""
+    lat = _lat(f)
+    rho, ux, uy, uz = macroscopic(f)
+    te = smagorinsky_tau(tau, pi_norm(f - equilibrium(lat, rho, ux, uy, uz)), rho, C_s * C_s)
+    return collide_mrt(f, tau, s_e, s_eps, s_q, s_pi, tau_cell=te)
+
+
+def collide_mrt(f: np.ndarray, tau: float, s_e=None, s_eps=None, s_q=None, s_pi=None, *,
+                tau_cell=None) -> np.ndarray:
     """``M⁻¹[m − S(m − m_eq)]`` (solver.py:249-335, solver3d.py:107-160, d3q27.py:643-726).
 
     Defaults: D2Q9 ``s_e=1.64, s_eps=1.54, s_q=1.7``; D3Q19/D3Q27
@@ -372,7 +383,11 @@ def collide_mrt(f: np.ndarray, tau: float, s_e=None, s_eps=None, s_q=None, s_pi=
     feq = equilibrium(lat, rho, ux, uy, uz).reshape(lat.q, -1)
     mom = m @ ff
     mom_eq = m @ feq
-    return (mi @ (mom - s[:, None] * (mom - mom_eq))).reshape(shape)
+    rates = np.broadcast_to(s[:, None], mom.shape).copy()
+    if tau_cell is not None:  # the 1/τ rows take the per-cell rate
+        shear = np.array(mrt_rates(lat, -1.0, 0.0, 0.0, 0.0, 0.0)) == -1.0
+        rates[shear] = 1.0 / np.asarray(tau_cell, dtype=dt).reshape(1, -1)
+    return (mi @ (mom - rates * (mom - mom_eq))).reshape(shape)
 
 
 # ---------------------------------------------------------------------------
@@ -870,3 +885,107 @@ def collide_cumulant_d2q9(f, tau, omega_b=1.0, omega_3=1.0, omega_4=1.0):
     A2 = np.moveaxis(A, (0, 1), (-2, -1))
     b2 = np.moveaxis(b, 0, -1)[..., None]
     return np.moveaxis(np.linalg.solve(A2, b2)[..., 0], -1, 0)
+
+
+# ---------------------------------------------------------------------------
+# legacy D3Q27 "cumulant" (collide_cumulant_d3q27)
+# ---------------------------------------------------------------------------
+
+
+def legacy_pseudo_cumulants(k: list) -> list:
+    """cumulant.py:470-577: Geier's partition terms of orders 4-6 without the 1/ρ factors,
+    applied to the central moments of f_neq; orders ≤ 3 pass through.  DEGREES_27 indices."""
+    c = list(k)
+    k200, k020, k002, k110, k101, k011 = k[4], k[5], k[6], k[7], k[8], k[9]
+    c[17] = k[17] - k200 * k020 - 2.0 * k110 * k110
+    c[18] = k[18] - k200 * k002 - 2.0 * k101 * k101
+    c[19] = k[19] - k020 * k002 - 2.0 * k011 * k011
+    c[20] = k[20] - k200 * k011 - 2.0 * k101 * k110
+    c[21] = k[21] - k020 * k101 - 2.0 * k110 * k011
+    c[22] = k[22] - k002 * k110 - 2.0 * k101 * k011
+    c[23] = k[23] - k200 * k[13] - 2.0 * k110 * k[16]
+    c[24] = k[24] - k200 * k[15] - 2.0 * k101 * k[16]
+    c[25] = k[25] - k020 * k[14] - 2.0 * k011 * k[16]
+    c[26] = (k[26] - k200 * c[19] - k020 * c[18] - k002 * c[17]
+             - 2.0 * (k110 * c[22] + k101 * c[21] + k011 * c[20])
+             + 2.0 * k200 * k020 * k002 + 4.0 * k110 * k101 * k011
+             + 2.0 * k110 * k110 * k002 + 2.0 * k101 * k101 * k020 + 2.0 * k011 * k011 * k200)
+    return c
+
+
+def legacy_central_moments(c: list) -> list:
+    """cumulant.py:580-651: the inverse map of ``legacy_pseudo_cumulants``."""
+    k = list(c)
+    a, b, d, xy, xz, yz = c[4], c[5], c[6], c[7], c[8], c[9]
+    k[17] = c[17] + a * b + 2.0 * xy * xy
+    k[18] = c[18] + a * d + 2.0 * xz * xz
+    k[19] = c[19] + b * d + 2.0 * yz * yz
+    k[20] = c[20] + a * yz + 2.0 * xz * xy
+    k[21] = c[21] + b * xz + 2.0 * xy * yz
+    k[22] = c[22] + d * xy + 2.0 * xz * yz
+    k[23] = c[23] + a * c[13] + 2.0 * xy * c[16]
+    k[24] = c[24] + a * c[15] + 2.0 * xz * c[16]
+    k[25] = c[25] + b * c[14] + 2.0 * yz * c[16]
+    k[26] = (c[26] + a * c[19] + b * c[18] + d * c[17] + 2.0 * (xy * c[22] + xz * c[21] + yz * c[20])
+             - 2.0 * a * b * d - 4.0 * xy * xz * yz - 2.0 * xy * xy * d - 2.0 * xz * xz * b - 2.0 * yz * yz * a)
+    return k
+
+
+def smagorinsky_tau_mean(f: np.ndarray, tau: float, C_s: float) -> float:
+    """Domain mean of the clamped per-cell Smagorinsky τ_eff (cumulant.py:986-997 before its clamp)."""
+    lat = _lat(f)
+    rho, ux, uy, uz = macroscopic(f)
+    fneq = f - equilibrium(lat, rho, ux, uy, uz)
+    return float(smagorinsky_tau(tau, pi_norm(fneq), rho, C_s * C_s).mean())
+
+
+def collide_cumulant_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
+    """cumulant.py:935-1047: f_neq raw moments → central → pseudo-cumulants → relax → back.
+    C_s > 0: ω from the domain-mean Smagorinsky τ_eff, clamped to [τ, 10 τ]."""
+    lat = _lat(f)
+    omega = 1.0 / tau
+    if C_s > 0:
+        te = smagorinsky_tau_mean(f, tau, C_s)
+        omega = 1.0 / max(tau, min(te, 10.0 * tau))
+    rho, ux, uy, uz = macroscopic(f)
+    feq = equilibrium(lat, rho, ux, uy, uz)
+    M, Mi = moment_matrix(lat, DEGREES_27)
+    sh = f.shape[1:]
+    m = list((M @ (f - feq).reshape(27, -1)).reshape((27,) + sh))
+    k = legacy_pseudo_cumulants(to_central(m, DEGREES_27, ux, uy, uz))
+    _relax_second(k, omega, omega_b)
+    for i in range(10, 17):
+        k[i] = (1.0 - omega_odd) * k[i]
+    for i in range(17, 27):
+        k[i] = (1.0 - omega_even) * k[i]
+    m = to_raw(legacy_central_moments(k), DEGREES_27, ux, uy, uz)
+    return feq + (Mi @ np.stack(m).reshape(27, -1)).reshape((27,) + sh)
+
+
+def far_field_bc(f: np.ndarray, u_in: float, obstacle: np.ndarray | None = None, uy: float = 0.0, uz: float = 0.0,
+                 bc_config: dict | None = None) -> np.ndarray:
+    """boundaries3d.py:446-537: free-stream equilibrium (ρ = 1) on the inlet x = 0 and the selected
+    lateral faces, copy outlet x = nx−1, then bounce-back in the obstacle; face order as there."""
+    lat = _lat(f)
+    one = np.ones((1, 1, 1))
+    feq = equilibrium(lat, one, one * u_in, one * uy, one * uz)[:, 0, 0, 0]
+    f = f.copy()
+    if bc_config is None:
+        ff, periodic = {"y-", "y+", "z-", "z+"}, set()
+    else:
+        ff, periodic = set(bc_config.get("far_field_faces", [])), set(bc_config.get("periodic_faces", []))
+    if "x-" not in periodic:
+        f[:, :, :, 0] = feq[:, None, None]
+    if "x+" not in periodic:
+        f[:, :, :, -1] = f[:, :, :, -2]
+    if "y-" in ff and "y-" not in periodic:
+        f[:, :, 0, :] = feq[:, None, None]
+    if "y+" in ff and "y+" not in periodic:
+        f[:, :, -1, :] = feq[:, None, None]
+    if "z-" in ff and "z-" not in periodic:
+        f[:, 0, :, :] = feq[:, None, None]
+    if "z+" in ff and "z+" not in periodic:
+        f[:, -1, :, :] = feq[:, None, None]
+    if obstacle is not None:
+        f = bounce_back(f, obstacle)
+    return f

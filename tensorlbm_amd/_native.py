@@ -30,7 +30,7 @@ D2Q9, D3Q19, D3Q27 = 9, 19, 27
 STORE_F32, STORE_F64, STORE_F16, STORE_BF16, STORE_F16_DEV, STORE_BF16_DEV = range(6)
 COLL_NONE, COLL_BGK, COLL_MRT, COLL_CM, COLL_CUMULANT, COLL_GEIER = range(6)
 TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP = range(5)
-BC_NONE, BC_CHANNEL_X, BC_LID_Y = range(3)
+BC_NONE, BC_CHANNEL_X, BC_LID_Y, BC_FAR_FIELD = range(4)
 
 
 class Collision(ctypes.Structure):
@@ -63,9 +63,30 @@ CAVITIES_MAX = 128
 class Boundary(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("faces", ctypes.c_int32),
         ("u", ctypes.c_double * 3),
     ]
+
+
+FACE_YM, FACE_YP, FACE_ZM, FACE_ZP, FACE_XM_PERIODIC, FACE_XP_PERIODIC = 1, 2, 4, 8, 16, 32
+_FACE_BITS = {"y-": FACE_YM, "y+": FACE_YP, "z-": FACE_ZM, "z+": FACE_ZP}
+
+
+def far_field_faces(bc_config: dict | None) -> int:
+    """TLBM_FACE_* bits of far_field_bc_3d's ``bc_config`` (boundaries3d.py:446-537; None: all four lateral faces)."""
+    if bc_config is None:
+        return FACE_YM | FACE_YP | FACE_ZM | FACE_ZP
+    ff = set(bc_config.get("far_field_faces", []))
+    periodic = set(bc_config.get("periodic_faces", []))
+    bits = 0
+    for face, bit in _FACE_BITS.items():
+        if face in ff and face not in periodic:
+            bits |= bit
+    if "x-" in periodic:
+        bits |= FACE_XM_PERIODIC
+    if "x+" in periodic:
+        bits |= FACE_XP_PERIODIC
+    return bits
 
 
 _VP = ctypes.c_void_p
@@ -86,6 +107,7 @@ _SIGNATURES = {
     "tlbm_moment_sums": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
     "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_scale": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_smagorinsky_tau_sum": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.c_double, ctypes.c_double, _VP, _VP], _I),
     "tlbm_mlp_sizes": ([_VP, _VP, _VP, _VP], _I),
     "tlbm_mlp_pack": ([_VP, _VP, _VP], _I),
     "tlbm_mlp_predict": ([_VP, _VP, _I64, _VP, _VP], _I),
@@ -230,9 +252,10 @@ def make_collision(
     return c
 
 
-def make_boundary(kind: int = BC_NONE, u=(0.0, 0.0, 0.0)) -> Boundary:
+def make_boundary(kind: int = BC_NONE, u=(0.0, 0.0, 0.0), faces: int = 0) -> Boundary:
     b = Boundary()
     b.kind = kind
+    b.faces = int(faces)
     for i in range(3):
         b.u[i] = float(u[i])
     return b

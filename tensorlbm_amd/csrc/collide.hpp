@@ -121,13 +121,17 @@ __device__ __forceinline__ C pi_norm(const C (&pi)[6]) {
 
 // Hou et al. tau_eff clamped to [0.5001, max(1, tau+0.5)]  (turbulence.py:183-208)
 template <typename C>
-__device__ __forceinline__ C smagorinsky_omega(C pn, C rho_safe, C tau, C cs_delta_sq, C tau_cap) {
+__device__ __forceinline__ C smagorinsky_tau(C pn, C rho_safe, C tau, C cs_delta_sq, C tau_cap) {
   C disc = tau * tau + C(18) * cs_delta_sq * pn / rho_safe;
   disc = disc > C(0) ? disc : C(0);
   C te = C(0.5) * (tau + sqrt(disc));
   te = te < C(0.5001) ? C(0.5001) : te;
-  te = te > tau_cap ? tau_cap : te;
-  return C(1) / te;
+  return te > tau_cap ? tau_cap : te;
+}
+
+template <typename C>
+__device__ __forceinline__ C smagorinsky_omega(C pn, C rho_safe, C tau, C cs_delta_sq, C tau_cap) {
+  return C(1) / smagorinsky_tau(pn, rho_safe, tau, cs_delta_sq, tau_cap);
 }
 
 // Unclamped closure of collide_cumulant_d3q19 (cumulant.py:1316-1321)
@@ -409,6 +413,74 @@ __device__ __forceinline__ void geier27(C (&f)[27], const C (&feq)[27], const Ma
   }
   to_raw<L>(m, mc.ux, mc.uy, mc.uz);
   populations<L>(m, f);
+}
+
+// Legacy D3Q27 "cumulant" of collide_cumulant_d3q27 (cumulant.py:935-1047):
+// central moments of f_neq pass through the pseudo-cumulant map of
+// cumulant.py:470-577 (Geier's partition terms without 1/rho; orders <= 3
+// unchanged), relax (trace / deviatoric at omega_b / omega, 3rd order at
+// omega_odd, 4th-6th at omega_even), map back (cumulant.py:580-651), unshift.
+// rate = {omega_b, omega_odd, omega_even}.  Index order: DEG27.
+template <typename C>
+__device__ __forceinline__ void legacy_to_pseudo27(C (&k)[27]) {
+  const C a = k[4], b = k[5], c = k[6], xy = k[7], xz = k[8], yz = k[9];
+  k[17] = k[17] - a * b - C(2) * xy * xy;
+  k[18] = k[18] - a * c - C(2) * xz * xz;
+  k[19] = k[19] - b * c - C(2) * yz * yz;
+  k[20] = k[20] - a * yz - C(2) * xz * xy;
+  k[21] = k[21] - b * xz - C(2) * xy * yz;
+  k[22] = k[22] - c * xy - C(2) * xz * yz;
+  k[23] = k[23] - a * k[13] - C(2) * xy * k[16];
+  k[24] = k[24] - a * k[15] - C(2) * xz * k[16];
+  k[25] = k[25] - b * k[14] - C(2) * yz * k[16];
+  // the 6th-order line reads the already mapped 4th-order entries
+  k[26] = k[26] - a * k[19] - b * k[18] - c * k[17] - C(2) * (xy * k[22] + xz * k[21] + yz * k[20]) +
+          C(2) * a * b * c + C(4) * xy * xz * yz + C(2) * xy * xy * c + C(2) * xz * xz * b + C(2) * yz * yz * a;
+}
+
+template <typename C>
+__device__ __forceinline__ void legacy_from_pseudo27(C (&k)[27]) {
+  const C a = k[4], b = k[5], c = k[6], xy = k[7], xz = k[8], yz = k[9];
+  // 6th order first: it reads the mapped (pseudo-cumulant) 4th-order entries
+  k[26] = k[26] + a * k[19] + b * k[18] + c * k[17] + C(2) * (xy * k[22] + xz * k[21] + yz * k[20]) -
+          C(2) * a * b * c - C(4) * xy * xz * yz - C(2) * xy * xy * c - C(2) * xz * xz * b - C(2) * yz * yz * a;
+  k[17] = k[17] + a * b + C(2) * xy * xy;
+  k[18] = k[18] + a * c + C(2) * xz * xz;
+  k[19] = k[19] + b * c + C(2) * yz * yz;
+  k[20] = k[20] + a * yz + C(2) * xz * xy;
+  k[21] = k[21] + b * xz + C(2) * xy * yz;
+  k[22] = k[22] + c * xy + C(2) * xz * yz;
+  k[23] = k[23] + a * k[13] + C(2) * xy * k[16];
+  k[24] = k[24] + a * k[15] + C(2) * xz * k[16];
+  k[25] = k[25] + b * k[14] + C(2) * yz * k[16];
+}
+
+template <typename C>
+__device__ __forceinline__ void legacy27(C (&f)[27], const C (&feq)[27], const Macro<C>& mc, C omega,
+                                         const C* rate) {
+  using L = D3Q27;
+  C v[27], m[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] = f[q] - feq[q];
+  moments<L>(v, m);
+  to_central<L>(m, mc.ux, mc.uy, mc.uz);
+  legacy_to_pseudo27(m);
+  const C tr = m[4] + m[5] + m[6];
+  const C trs = (C(1) - rate[0]) * tr;
+  const C third = C(1.0 / 3.0);
+#pragma unroll
+  for (int i = 4; i < 7; ++i) m[i] = (C(1) - omega) * (m[i] - tr * third) + trs * third;
+#pragma unroll
+  for (int i = 7; i < 10; ++i) m[i] = (C(1) - omega) * m[i];
+#pragma unroll
+  for (int i = 10; i < 17; ++i) m[i] = (C(1) - rate[1]) * m[i];
+#pragma unroll
+  for (int i = 17; i < 27; ++i) m[i] = (C(1) - rate[2]) * m[i];
+  legacy_from_pseudo27(m);
+  to_raw<L>(m, mc.ux, mc.uy, mc.uz);
+  populations<L>(m, v);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) f[q] = feq[q] + v[q];
 }
 
 }  // namespace tlbm

@@ -51,7 +51,7 @@ struct KArgs {
   int64_t isq, isz, isy;  // input strides (elements; x stride 1)
   int64_t osq, osz, osy;  // output strides
   int nx, ny, nz, tiles_x, tiles_y;
-  int tau_mode, eq_floor, bc_kind;
+  int tau_mode, eq_floor, bc_kind, bc_faces;
   C omega, tau, cs_delta_sq, tau_cap;
   C rate[8];
   C s[27];
@@ -175,6 +175,19 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
       }
     } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1) {
       if constexpr (Q == 19) zou_he_lid(f, a.bc_u[0]);
+    } else if (STREAM && a.bc_kind == TLBM_BC_FAR_FIELD) {
+      // far_field_bc_3d (boundaries3d.py:446-537): free-stream equilibrium on the
+      // inlet and the selected lateral faces (lateral faces win on edges),
+      // zero-gradient outlet
+      const int fm = a.bc_faces;
+      const bool lateral = (yy == 0 && (fm & TLBM_FACE_YM)) || (yy == ny - 1 && (fm & TLBM_FACE_YP)) ||
+                           (z == 0 && (fm & TLBM_FACE_ZM)) || (z == nz - 1 && (fm & TLBM_FACE_ZP));
+      if (lateral || (xx == 0 && !(fm & TLBM_FACE_XM_PERIODIC))) {
+        equilibrium<L>(C(1), a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
+      } else if (xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = pull_one<ST, L>(fin, a, q, nx - 2, yy, z);
+      }
     }
     if (solid) {
 #pragma unroll
@@ -192,7 +205,7 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
     const Macro<C> mc = macro<L>(f);
     const C rho_eq = a.eq_floor ? mc.rho_safe : mc.rho;
     C omega = a.omega;
-    const bool need_pi = (COLL == TLBM_COLL_CUMULANT) || a.tau_mode == TLBM_TAU_SMAG ||
+    const bool need_pi = (COLL == TLBM_COLL_CUMULANT && Q == 19) || a.tau_mode == TLBM_TAU_SMAG ||
                          a.tau_mode == TLBM_TAU_SMAG_RAW;
     if constexpr (COLL == TLBM_COLL_BGK && kMlp) {
       // learned closure: strain from the non-equilibrium stress with the
@@ -250,7 +263,8 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
       } else if constexpr (COLL == TLBM_COLL_CM) {
         cascaded<L>(f, feq, mc, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
-        cumulant19<L>(f, feq, pi, omega, a.rate);
+        if constexpr (Q == 19) cumulant19<L>(f, feq, pi, omega, a.rate);
+        if constexpr (Q == 27) legacy27<C>(f, feq, mc, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_GEIER) {
         if constexpr (Q == 27) geier27<C>(f, feq, mc, omega, a.rate);
       }
@@ -376,6 +390,46 @@ __global__ __launch_bounds__(256) void k_moment_sums(const void* fv, Layout lay,
     } else if (threadIdx.x > 0) {
       atomicAdd(out + threadIdx.x - 1, 2.0 * t);
     }
+  }
+}
+
+// sum over all cells of the clamped Smagorinsky tau_eff (turbulence.py:183-208) of
+// the local non-equilibrium stress, f_eq built with rho (the eager ops' form)
+template <class L, int S>
+__global__ __launch_bounds__(256) void k_tau_sum(const void* fv, Layout lay, double tau, double cs_delta_sq,
+                                                 double* out) {
+  using ST = Store<S>;
+  using C = typename ST::C;
+  const typename ST::T* f = static_cast<const typename ST::T*>(fv);
+  double acc = 0.0;
+  const uint64_t N = uint64_t(lay.nx) * uint64_t(lay.ny) * uint64_t(lay.nz);
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const C tau_cap = C(tau + 0.5 > 1.0 ? tau + 0.5 : 1.0);
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N; i += stride) {
+    const uint64_t x = i % uint64_t(lay.nx);
+    const uint64_t r = i / uint64_t(lay.nx);
+    const uint64_t y = r % uint64_t(lay.ny);
+    const uint64_t z = r / uint64_t(lay.ny);
+    const int64_t base = int64_t(z) * lay.sz + int64_t(y) * lay.sy + int64_t(x);
+    C v[L::Q], feq[L::Q], pi[6];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) v[q] = ST::template get<L>(f[base + int64_t(q) * lay.sq], q);
+    const Macro<C> mc = macro<L>(v);
+    equilibrium<L>(mc.rho, mc.ux, mc.uy, mc.uz, feq);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) v[q] -= feq[q];
+    neq_stress<L>(v, pi);
+    acc += double(smagorinsky_tau(pi_norm(pi), mc.rho_safe, C(tau), C(cs_delta_sq), tau_cap));
+  }
+  acc = wave_sum(acc);
+  __shared__ double part[4];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  if (lane == 0) part[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < int(blockDim.x / 64); ++w) t += part[w];
+    atomicAdd(out, t);
   }
 }
 
@@ -550,15 +604,21 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
     }
   }
   a.bc_kind = TLBM_BC_NONE;
+  a.bc_faces = 0;
   if (bc != nullptr && bc->kind != TLBM_BC_NONE) {
     if (bc->kind == TLBM_BC_CHANNEL_X) {
       if (nx < 3) return fail(TLBM_ERR_SHAPE, "channel boundary needs nx >= 3");
     } else if (bc->kind == TLBM_BC_LID_Y) {
       if (lattice != TLBM_D3Q19) return fail(TLBM_ERR_UNSUPPORTED, "moving lid is D3Q19 only");
+    } else if (bc->kind == TLBM_BC_FAR_FIELD) {
+      if (lattice == TLBM_D2Q9) return fail(TLBM_ERR_UNSUPPORTED, "far-field faces are 3-D only");
+      if (nx < 3) return fail(TLBM_ERR_SHAPE, "far-field boundary needs nx >= 3");
+      if (bc->faces & ~63) return fail(TLBM_ERR_INVALID, "unknown far-field face bits 0x%x", bc->faces);
     } else {
       return fail(TLBM_ERR_INVALID, "unknown boundary kind %d", bc->kind);
     }
     a.bc_kind = bc->kind;
+    a.bc_faces = bc->faces;
     for (int i = 0; i < 3; ++i) a.bc_u[i] = C(bc->u[i]);
   }
   return TLBM_OK;
@@ -577,7 +637,7 @@ template <class L, int COLL>
 constexpr bool coll_built() {
   if (COLL == TLBM_COLL_NONE || COLL == TLBM_COLL_BGK || COLL == TLBM_COLL_MRT) return true;
   if (COLL == TLBM_COLL_CM) return true;
-  if (COLL == TLBM_COLL_CUMULANT) return L::Q == 19;
+  if (COLL == TLBM_COLL_CUMULANT) return L::Q != 9;
   if (COLL == TLBM_COLL_GEIER) return L::Q == 27;
   return false;
 }
@@ -905,6 +965,31 @@ int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* st
   g_err[0] = 0;
   if (!solid) return fail(TLBM_ERR_INVALID, "null solid mask");
   return launch_sums(lattice, store, f, strides, nz, ny, nx, solid, out, (hipStream_t)stream);
+}
+
+int tlbm_smagorinsky_tau_sum(int lattice, int store, const void* f, const int64_t* strides, int nz, int ny, int nx,
+                             double tau, double cs_delta_sq, double* out, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  int rc = check_dims(lattice, nz, ny, nx);
+  if (rc) return rc;
+  if (!store_ok(store)) return fail(TLBM_ERR_DTYPE, "unknown storage %d", store);
+  if (!f || !out) return fail(TLBM_ERR_INVALID, "null pointer");
+  if (!(tau > 0.5)) return fail(TLBM_ERR_INVALID, "tau must be > 0.5 (got %g)", tau);
+  int64_t st[3];
+  rc = resolve_strides(strides, lattice, nz, ny, nx, store_size(store), st, "f");
+  if (rc) return rc;
+  const Layout lay{st[0], st[1], st[2], nx, ny, nz};
+  uint32_t nb = blocks_for(uint64_t(nz) * ny * nx);
+  if (nb > 4096) nb = 4096;
+  hipStream_t s = (hipStream_t)stream;
+  return by_lattice(lattice, [&](auto lat) -> int {
+    using L = decltype(lat);
+    return by_store<L>(store, [&](auto sk) -> int {
+      constexpr int S = decltype(sk)::value;
+      hipLaunchKernelGGL((k_tau_sum<L, S>), dim3(nb), dim3(256), 0, s, f, lay, tau, cs_delta_sq, out);
+      return check_launch("k_tau_sum");
+    });
+  });
 }
 
 int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx, const double* scale,

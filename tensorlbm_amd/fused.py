@@ -219,7 +219,8 @@ class FusedSolver3D:
     Extra keywords select the lattice (``"D3Q19"`` default, ``"D2Q9"``,
     ``"D3Q27"``), the collision family, deviation storage for 16-bit dtypes,
     an optional solid mask (full-way bounce-back) and a boundary
-    (``"channel_x"`` with ``u_bc`` = inlet velocity, or ``"lid_y"``).
+    (``"channel_x"`` with ``u_bc`` = inlet velocity, ``"lid_y"``, or
+    ``"far_field"`` with ``u_bc`` = free stream and far_field_bc_3d's ``bc_config``).
     ``closure=EddyViscosityMLP3D`` relaxes D3Q19 BGK with the learned eddy
     viscosity, evaluated inside the step (``C_s`` then sets the strain's time
     scale: the Smagorinsky τ of that constant, or τ itself for ``C_s=0``).
@@ -230,7 +231,7 @@ class FusedSolver3D:
                  collision: str = "BGK", rates: Sequence[float] | None = None, deviation: bool = False,
                  C_s: float = 0.0, delta: float = 1.0, solid: torch.Tensor | None = None,
                  boundary: str | None = None, u_bc=(0.0, 0.0, 0.0), layout: str = "padded",
-                 closure=None, **_tiles) -> None:
+                 closure=None, bc_config: dict | None = None, **_tiles) -> None:
         if precision is not None:
             policy = PrecisionPolicy.parse(precision)
             if policy.compute_dtype != torch.float32:
@@ -281,10 +282,12 @@ class FusedSolver3D:
             if tuple(solid.shape) != (self.nz, self.ny, self.nx) and not (lat == "D2Q9" and tuple(solid.shape) == (self.ny, self.nx)):
                 raise ValueError("solid mask shape must match the grid")
             self.solid = solid.to(device=self.device, dtype=torch.int8).contiguous()
-        kinds = {None: nat.BC_NONE, "channel_x": nat.BC_CHANNEL_X, "lid_y": nat.BC_LID_Y}
+        kinds = {None: nat.BC_NONE, "channel_x": nat.BC_CHANNEL_X, "lid_y": nat.BC_LID_Y,
+                 "far_field": nat.BC_FAR_FIELD}
         if boundary not in kinds:
-            raise ValueError(f"unknown boundary {boundary!r}; valid: channel_x, lid_y")
-        self._bc = nat.make_boundary(kinds[boundary], u_bc) if boundary is not None else None
+            raise ValueError(f"unknown boundary {boundary!r}; valid: channel_x, lid_y, far_field")
+        faces = nat.far_field_faces(bc_config) if boundary == "far_field" else 0
+        self._bc = nat.make_boundary(kinds[boundary], u_bc, faces) if boundary is not None else None
         if layout not in ("padded", "contiguous"):
             raise ValueError(f"layout must be 'padded' or 'contiguous', got {layout!r}")
         self.layout = layout

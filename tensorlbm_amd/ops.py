@@ -32,8 +32,10 @@ __all__ = [
     "equilibrium", "macroscopic", "equilibrium3d", "equilibrium3d_low_memory", "macroscopic3d",
     "macroscopic3d_low_memory", "equilibrium27", "macroscopic27", "collide_bgk", "collide_bgk3d",
     "collide_bgk27", "collide_mrt", "collide_mrt3d", "collide_mrt27", "collide_cascaded_d3q19",
-    "collide_cascaded_d3q27", "collide_cumulant_d2q9", "collide_cumulant_d3q19", "collide_cumulant_geier_d3q27",
-    "collide_smagorinsky_bgk", "collide_smagorinsky_bgk3d", "collide_smagorinsky_bgk27", "stream",
+    "collide_cascaded_d3q27", "collide_cumulant_d2q9", "collide_cumulant_d3q19", "collide_cumulant_d3q27",
+    "collide_cumulant_geier_d3q27",
+    "collide_smagorinsky_bgk", "collide_smagorinsky_bgk3d", "collide_smagorinsky_bgk27", "collide_smagorinsky_mrt",
+    "collide_smagorinsky_mrt3d", "stream",
     "stream3d", "stream3d_roll", "stream27", "stream27_roll", "correct_mass", "correct_mass3d",
     "correct_mass27", "bounce_back_cells", "bounce_back_cells_3d", "bounce_back_cells_27",
     "compute_obstacle_forces_3d", "moment_sums",
@@ -275,6 +277,28 @@ def collide_cumulant_d3q19(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C
     return _collide(f, 19, coll, "collide_cumulant_d3q19")
 
 
+def collide_cumulant_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
+    """Legacy D3Q27 central-moment operator with the pseudo-cumulant map on f_neq (cumulant.py:935-1047).
+
+    ``C_s > 0`` relaxes every cell with one rate: the domain mean of the per-cell clamped
+    Smagorinsky τ_eff, clamped to [τ, 10 τ] (cumulant.py:986-997).  The mean is an FP64 device
+    reduction read back on the host, as the reference's ``.item()``.
+    """
+    if f.ndim != 4 or f.shape[0] != 27:
+        raise ValueError(f"D3Q27 populations must have shape (27, nz, ny, nx), got {tuple(f.shape)}")
+    tau = _tau(tau, "collide_cumulant_d3q27")
+    if C_s > 0:
+        fc, dims = _pops(f, 27, "collide_cumulant_d3q27")
+        acc = torch.zeros(1, dtype=torch.float64, device=fc.device)
+        nat.check(nat.lib().tlbm_smagorinsky_tau_sum(27, _eager_dtype(fc, "collide_cumulant_d3q27"), fc.data_ptr(),
+                                                     None, *dims, tau, float(C_s) ** 2, acc.data_ptr(),
+                                                     nat.stream_of(fc)), "collide_cumulant_d3q27")
+        te = float(acc.item()) / (dims[0] * dims[1] * dims[2])
+        tau = max(tau, min(te, 10.0 * tau))
+    coll = nat.make_collision(nat.COLL_CUMULANT, tau, eq_rho_floor=False, rates=(omega_b, omega_odd, omega_even))
+    return _collide(f, 27, coll, "collide_cumulant_d3q27")
+
+
 def collide_cumulant_geier_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
     """Geier 2015 cumulant collision (cumulant.py:1050-1156).
 
@@ -307,6 +331,21 @@ def collide_smagorinsky_bgk3d(f, tau, C_s=0.1):
 def collide_smagorinsky_bgk27(f, tau, C_s=0.1):
     """D3Q27 BGK with clamped Smagorinsky τ_eff (turbulence.py:472-494)."""
     return _smag(f, 27, tau, C_s, "collide_smagorinsky_bgk27")
+
+
+def collide_smagorinsky_mrt(f, tau, C_s=0.1, s_e=1.64, s_eps=1.54, s_q=1.7):
+    """D2Q9 MRT whose stress moments relax at the per-cell Smagorinsky 1/τ_eff (turbulence.py:270-342)."""
+    coll = nat.make_collision(nat.COLL_MRT, _tau(tau, "collide_smagorinsky_mrt"), tau_mode=nat.TAU_SMAG,
+                              eq_rho_floor=False, cs_delta_sq=float(C_s) ** 2, rates=(s_e, s_eps, s_q, s_e))
+    return _collide(f, 9, coll, "collide_smagorinsky_mrt")
+
+
+def collide_smagorinsky_mrt3d(f, tau, C_s=0.1, s_e=1.19, s_eps=1.4, s_q=1.2, s_pi=None):
+    """D3Q19 MRT whose stress moments 9-13 relax at the per-cell Smagorinsky 1/τ_eff (turbulence.py:345-433)."""
+    s_pi = s_e if s_pi is None else s_pi
+    coll = nat.make_collision(nat.COLL_MRT, _tau(tau, "collide_smagorinsky_mrt3d"), tau_mode=nat.TAU_SMAG,
+                              eq_rho_floor=False, cs_delta_sq=float(C_s) ** 2, rates=(s_e, s_eps, s_q, s_pi))
+    return _collide(f, 19, coll, "collide_smagorinsky_mrt3d")
 
 
 # ---------------------------------------------------------------------------

@@ -126,6 +126,39 @@ def test_cumulant_d3q19(cs, dtype, dev):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("cs", [0.0, 0.12])
+def test_cumulant_d3q27_legacy(cs, dtype, dev):
+    f = _state(27, noise=3e-2)
+    got = tl.collide_cumulant_d3q27(_gpu(f, dev, dtype), 0.83, omega_b=1.13, omega_odd=0.71, omega_even=1.37, C_s=cs)
+    want = ref.collide_cumulant_d3q27(f, 0.83, 1.13, 0.71, 1.37, C_s=cs)
+    _cmp(got, want, dtype)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_smagorinsky_tau_sum(q, dev):
+    from tensorlbm_amd import _native as nat
+
+    f = _state(q, noise=5e-2)
+    acc = torch.zeros(1, dtype=torch.float64, device=dev)
+    fg = _gpu(f, dev, torch.float64)
+    nat.check(nat.lib().tlbm_smagorinsky_tau_sum(q, nat.STORE_F64, fg.data_ptr(), None, *f.shape[1:], 0.6, 0.09,
+                                                 acc.data_ptr(), nat.stream_of(fg)), "tau sum")
+    want = ref.smagorinsky_tau_mean(f, 0.6, 0.3) * f[0].size
+    assert abs(float(acc) - want) < 1e-10 * want
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("q", [9, 19])
+def test_smagorinsky_mrt(q, dtype, dev):
+    f = _state(q, noise=5e-2)
+    if q == 9:
+        got = tl.collide_smagorinsky_mrt(_gpu(f, dev, dtype), 0.58, C_s=0.2)
+    else:
+        got = tl.collide_smagorinsky_mrt3d(_gpu(f, dev, dtype), 0.58, C_s=0.2, s_pi=1.3)
+    _cmp(got, ref.collide_smagorinsky_mrt(f, 0.58, 0.2, s_pi=None if q == 9 else 1.3), dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_geier_d3q27(dtype, dev):
     f = _state(27)
     got = tl.collide_cumulant_geier_d3q27(_gpu(f, dev, dtype), 0.58, omega_b=1.1, omega_odd=1.3, omega_even=0.9)

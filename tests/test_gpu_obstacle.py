@@ -121,3 +121,23 @@ def test_channel_sphere_multistep_vs_production_chain(dev):
     want = ref.collide_bgk(h, tau)
     got = b.cpu().numpy()
     assert np.allclose(got, want, rtol=1e-11, atol=1e-13), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("bc_config", [None, {"far_field_faces": ["y-", "z+"], "periodic_faces": ["x+"]},
+                                       {"far_field_faces": ["y+", "z-"], "periodic_faces": ["x-", "z-"]}])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_far_field_matches_oracle_chain(dev, bc_config, dtype):
+    """stream → far_field_bc_3d (boundaries3d.py:446-537) → bounce-back → BGK, in one launch."""
+    solid, f0 = _case(noise=1e-2)
+    tau, uin = 0.56, (0.05, 0.01, -0.02)
+    s = tl.FusedSolver3D(NZ, NY, NX, tau, dev, dtype, solid=torch.tensor(solid, device=dev), boundary="far_field",
+                         u_bc=uin, bc_config=bc_config, layout="contiguous")
+    f = _gpu(f0, dev, dtype)
+    want = f0.astype(np.float64)
+    for _ in range(3):
+        f = s.step(f)
+        fin = ref.far_field_bc(ref.stream(want), uin[0], solid, uin[1], uin[2], bc_config)
+        feq, _, _ = ref._feq_safe(fin)
+        want = fin - (fin - feq) / tau
+    tol = dict(rtol=1e-5, atol=1e-7) if dtype == torch.float32 else dict(rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(f.cpu().numpy(), want, **tol)

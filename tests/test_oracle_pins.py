@@ -526,3 +526,94 @@ def test_cumulant_d2q9_relaxes_each_mode_at_its_rate():
     dv = lambda g: kap(g, 2, 0) - kap(g, 0, 2)  # noqa: E731
     np.testing.assert_allclose(tr(out) - tr(feq), (1 - ob) * (tr(f) - tr(feq)), atol=1e-15)
     np.testing.assert_allclose(dv(out) - dv(feq), (1 - 1 / tau) * (dv(f) - dv(feq)), atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# legacy D3Q27 "cumulant" (collide_cumulant_d3q27)
+# ---------------------------------------------------------------------------
+
+
+def _central27(f):
+    """Brute-force central moments Σ f (c − u)^α over DEGREES_27."""
+    rho, ux, uy, uz = ref.macroscopic(f)
+    c = D3Q27.c.astype(float).reshape(27, 3, *([1] * (f.ndim - 1)))
+    return [(f * (c[:, 0] - ux) ** a * (c[:, 1] - uy) ** b * (c[:, 2] - uz) ** d).sum(0)
+            for a, b, d in ref.DEGREES_27]
+
+
+def test_legacy_map_round_trip():
+    rng = np.random.default_rng(21)
+    k = list(rng.standard_normal((27, 5)))
+    back = ref.legacy_central_moments(ref.legacy_pseudo_cumulants(k))
+    np.testing.assert_allclose(np.stack(back), np.stack(k), rtol=0, atol=1e-13)
+
+
+def test_legacy_sixth_order_of_an_equilibrium_is_one_ninth():
+    """Golden value of the reference's test_cumulant_d3q27_geier.py:160-183: the legacy map of a
+    full equilibrium (ρ = 1, u_x = 0.1) leaves C_222 = 1/9 where Geier's transform gives 0."""
+    sh = (2, 2, 2)
+    feq = ref.equilibrium(D3Q27, np.ones(sh), np.full(sh, 0.1), np.zeros(sh), np.zeros(sh))
+    k = _central27(feq)
+    assert np.allclose(ref.legacy_pseudo_cumulants(k)[26], 1.0 / 9.0, rtol=0, atol=1e-6)
+    assert np.abs(ref.central_to_cumulant(k)[26]).max() < 1e-12
+
+
+def test_legacy_fourth_order_block_is_geiers_at_unit_density():
+    rng = np.random.default_rng(22)
+    k = list(rng.standard_normal((27, 4)) * 0.1)
+    k[0] = np.ones(4)
+    k[1] = k[2] = k[3] = np.zeros(4)
+    leg, gei = ref.legacy_pseudo_cumulants(k), ref.central_to_cumulant(k)
+    for i in range(4, 23):
+        np.testing.assert_allclose(leg[i], gei[i], rtol=0, atol=1e-14, err_msg=str(ref.DEGREES_27[i]))
+
+
+def test_legacy_operator_invariants_and_linear_limit():
+    f = F.random_state(27, (2, 3, 4), seed=23, noise=1e-3)
+    tau, ob, oo, oe = 0.83, 1.13, 0.71, 1.37
+    out = ref.collide_cumulant_d3q27(f, tau, ob, oo, oe)
+    for a, b in zip(ref.macroscopic(out), ref.macroscopic(f)):
+        np.testing.assert_allclose(a, b, atol=1e-15)
+    rho, ux, uy, uz = ref.macroscopic(f)
+    feq = ref.equilibrium(D3Q27, rho, ux, uy, uz)
+    np.testing.assert_allclose(ref.collide_cumulant_d3q27(feq, tau, ob, oo, oe), feq, atol=1e-15)
+    np.testing.assert_allclose(ref.collide_cumulant_d3q27(f, 1.0), feq, atol=1e-15)
+    # the map's corrections are quadratic in f_neq: at small f_neq the operator is the
+    # cascaded central-moment collision with the same rates
+    for eps, bound in ((1e-4, 1e-9), (1e-5, 1e-11)):
+        g = feq + eps * (f - feq) / 1e-3
+        d = ref.collide_cumulant_d3q27(g, tau, ob, oo, oe) - ref.collide_cascaded(g, tau, ob, oo, oe, oe, oe)
+        assert np.abs(d).max() < bound, (eps, np.abs(d).max())
+    # and it is not the cascaded operator at finite f_neq
+    g = feq + 30.0 * (f - feq)
+    d = ref.collide_cumulant_d3q27(g, tau, ob, oo, oe) - ref.collide_cascaded(g, tau, ob, oo, oe, oe, oe)
+    assert np.abs(d).max() > 1e-10
+
+
+def test_legacy_smagorinsky_mean_tau_is_clamped():
+    f = F.random_state(27, (2, 3, 4), seed=24, noise=5e-2)
+    tau = 0.6
+    te = ref.smagorinsky_tau_mean(f, tau, 0.3)
+    assert tau < te < ref.tau_eff_max(tau)
+    out = ref.collide_cumulant_d3q27(f, tau, C_s=0.3)
+    np.testing.assert_allclose(out, ref.collide_cumulant_d3q27(f, te), atol=1e-15)
+
+
+@pytest.mark.parametrize("lat", (D2Q9, D3Q19), ids=lambda l: l.name)
+def test_smagorinsky_mrt_uniform_field_equals_mrt_and_conserves(lat):
+    q = lat.q
+    f = F.random_state(q, (1, 5, 6) if q == 9 else (3, 4, 5), seed=31, noise=4e-2)
+    if q == 9:
+        f = f[:, 0]
+    tau = 0.58
+    out = ref.collide_smagorinsky_mrt(f, tau, 0.2)
+    for a, b in zip(ref.macroscopic(out), ref.macroscopic(f)):
+        np.testing.assert_allclose(a, b, atol=1e-15)
+    # with C_s = 0 the per-cell rate is 1/τ everywhere: plain MRT
+    np.testing.assert_allclose(ref.collide_smagorinsky_mrt(f, tau, 0.0), ref.collide_mrt(f, tau), atol=1e-15)
+    # per cell, the result is MRT at that cell's τ_eff
+    rho, ux, uy, uz = ref.macroscopic(f)
+    te = ref.smagorinsky_tau(tau, ref.pi_norm(f - ref.equilibrium(lat, rho, ux, uy, uz)), rho, 0.04)
+    idx = (slice(None), 2, 3) if q == 9 else (slice(None), 1, 2, 3)
+    cell = f[(slice(None),) + tuple(slice(i, i + 1) for i in idx[1:])]
+    np.testing.assert_allclose(out[idx], ref.collide_mrt(cell, float(te[idx[1:]])).reshape(-1), atol=1e-15)
