@@ -99,15 +99,26 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def split_bf16(a):
+    """Two-term bf16 split a ≈ hi + lo, hi = bf16(a), lo = bf16(a − hi), of fp32 values."""
+    a32 = np.asarray(a, np.float32)
+    hi = bf16(a32)
+    lo = bf16((a32.astype(np.float64) - hi).astype(np.float32))
+    return hi, lo
+
+
 def forward_bf16(vec, x, mean, inv_std):
-    """ν_t as the fused collision evaluates it: bf16 normalised inputs, W1, W2 and layer-1
-    activations, wide accumulation, fp32 output layer (the matrix-core path of the kernel)."""
+    """ν_t as the fused collision evaluates it on the matrix cores: bf16 W1 and W2, normalised
+    inputs and layer-1 activations as two-term bf16 splits, wide accumulation, fp32 output layer."""
     p = unpack(np.asarray(vec, dtype=np.float64))
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
-    xh = bf16((f32(x) - f32(mean)) * f32(inv_std))
-    z1 = xh @ bf16(p["w1"]).T + f32(p["b1"])
-    h1 = bf16(np.maximum(z1, 0.0))
-    z2 = h1 @ bf16(p["w2"]).T + f32(p["b2"])
+    xh = f32((f32(x) - f32(mean)) * f32(inv_std))
+    xhi, xlo = split_bf16(xh)
+    w1 = bf16(p["w1"])
+    z1 = xhi @ w1.T + xlo @ w1.T + f32(p["b1"])
+    hhi, hlo = split_bf16(np.maximum(z1, 0.0))
+    w2 = bf16(p["w2"])
+    z2 = hhi @ w2.T + hlo @ w2.T + f32(p["b2"])
     h2 = np.maximum(z2, 0.0)
     z3 = h2 @ f32(p["w3"]) + f32(p["b3"])[0]
     return softplus(z3)

@@ -660,6 +660,14 @@ def _factorial(n: int) -> int:
     return r
 
 
+def collide_cumulant_geier_d3q27_les(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
+    """cumulant.py:1101-1110: with C_s > 0 the rate comes from the domain-mean Smagorinsky τ_eff
+    clamped to [τ, 10 τ], as in the legacy operator."""
+    if C_s > 0:
+        tau = max(tau, min(smagorinsky_tau_mean(f, tau, C_s), 10.0 * tau))
+    return collide_cumulant_geier_d3q27(f, tau, omega_b, omega_odd, omega_even)
+
+
 def collide_cumulant_geier_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, *, omega=None):
     """Geier 2015 cumulant collision on the full f (cumulant.py:1050-1156).
 

@@ -34,9 +34,7 @@ __global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ p, Pa
   const int l = threadIdx.x;
   const int h = l >> 5, i = l & 31;
   for (int e = 0; e < 8; ++e) {
-    const float w = e < NIN ? p[OW1 + i * NIN + e] : 0.0f;
-    out->a1lo[l][e] = h == 0 ? bf16_bits(w) : uint16_t(0);
-    out->a1hi[l][e] = h == 1 ? bf16_bits(w) : uint16_t(0);
+    out->a1[l][e] = e < NIN ? bf16_bits(p[OW1 + i * NIN + e]) : uint16_t(0);
     for (int s = 0; s < 2; ++s) out->a2[s][l][e] = bf16_bits(p[OW2 + i * NH + w2_col(s, h, e)]);
   }
   if (l < 32) {

@@ -95,8 +95,12 @@ __device__ __forceinline__ typename ST::C pull_one(const typename ST::T* fin, co
 
 extern __shared__ __align__(16) unsigned char dyn_lds[];
 
+#ifndef TLBM_MLP_WAVES
+#define TLBM_MLP_WAVES 4
+#endif
+
 template <class L, int S, int COLL, bool STREAM, int FEAT>
-__global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
+__device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   using ST = Store<S>;
   using T = typename ST::T;
   using C = typename ST::C;
@@ -292,6 +296,20 @@ __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a
       }
     }
   }
+}
+
+template <class L, int S, int COLL, bool STREAM, int FEAT>
+__global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
+  lbm_body<L, S, COLL, STREAM, FEAT>(a);
+}
+
+// The learned-closure variants hold the matrix-core tiles next to the Q
+// populations; the occupancy floor keeps enough waves in flight to cover HBM
+// latency.
+template <class L, int S, int COLL, bool STREAM, int FEAT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TLBM_MLP_WAVES))) void k_lbm_mlp(
+    const KArgs<typename Store<S>::C> a) {
+  lbm_body<L, S, COLL, STREAM, FEAT>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,8 +646,11 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
 
 template <class L, int S, int COLL, bool STREAM, int FEAT>
 int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
-  const size_t lds = (FEAT & FEAT_MLP) ? sizeof(mlp::Packed) : 0;
-  hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), lds, s, a);
+  if constexpr ((FEAT & FEAT_MLP) != 0) {
+    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), sizeof(mlp::Packed), s, a);
+  } else {
+    hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
+  }
   return check_launch("k_lbm");
 }
 

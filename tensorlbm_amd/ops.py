@@ -288,27 +288,33 @@ def collide_cumulant_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C
         raise ValueError(f"D3Q27 populations must have shape (27, nz, ny, nx), got {tuple(f.shape)}")
     tau = _tau(tau, "collide_cumulant_d3q27")
     if C_s > 0:
-        fc, dims = _pops(f, 27, "collide_cumulant_d3q27")
-        acc = torch.zeros(1, dtype=torch.float64, device=fc.device)
-        nat.check(nat.lib().tlbm_smagorinsky_tau_sum(27, _eager_dtype(fc, "collide_cumulant_d3q27"), fc.data_ptr(),
-                                                     None, *dims, tau, float(C_s) ** 2, acc.data_ptr(),
-                                                     nat.stream_of(fc)), "collide_cumulant_d3q27")
-        te = float(acc.item()) / (dims[0] * dims[1] * dims[2])
-        tau = max(tau, min(te, 10.0 * tau))
+        tau = _domain_mean_smagorinsky_tau(f, tau, C_s, "collide_cumulant_d3q27")
     coll = nat.make_collision(nat.COLL_CUMULANT, tau, eq_rho_floor=False, rates=(omega_b, omega_odd, omega_even))
     return _collide(f, 27, coll, "collide_cumulant_d3q27")
+
+
+def _domain_mean_smagorinsky_tau(f, tau, C_s, what):
+    """Mean over the grid of the per-cell clamped Smagorinsky τ_eff, clamped to [τ, 10 τ] (cumulant.py:986-997)."""
+    fc, dims = _pops(f, 27, what)
+    acc = torch.zeros(1, dtype=torch.float64, device=fc.device)
+    nat.check(nat.lib().tlbm_smagorinsky_tau_sum(27, _eager_dtype(fc, what), fc.data_ptr(), None, *dims, tau,
+                                                 float(C_s) ** 2, acc.data_ptr(), nat.stream_of(fc)), what)
+    te = float(acc.item()) / (dims[0] * dims[1] * dims[2])
+    return max(tau, min(te, 10.0 * tau))
 
 
 def collide_cumulant_geier_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
     """Geier 2015 cumulant collision (cumulant.py:1050-1156).
 
-    The domain-averaged Smagorinsky option (``C_s > 0``) needs a host
-    round trip in the reference and is not provided here.
+    ``C_s > 0``: one rate from the domain-mean Smagorinsky τ_eff, the legacy operator's policy
+    (cumulant.py:1101-1110), read back on the host like the reference's ``.item()``.
     """
+    if f.ndim != 4 or f.shape[0] != 27:
+        raise ValueError(f"D3Q27 populations must have shape (27, nz, ny, nx), got {tuple(f.shape)}")
+    tau = _tau(tau, "collide_cumulant_geier_d3q27")
     if C_s > 0.0:
-        raise NotImplementedError("domain-averaged Smagorinsky (C_s > 0) is not on the HIP path")
-    coll = nat.make_collision(nat.COLL_GEIER, _tau(tau, "collide_cumulant_geier_d3q27"), eq_rho_floor=False,
-                              rates=(omega_b, omega_odd, omega_even))
+        tau = _domain_mean_smagorinsky_tau(f, tau, C_s, "collide_cumulant_geier_d3q27")
+    coll = nat.make_collision(nat.COLL_GEIER, tau, eq_rho_floor=False, rates=(omega_b, omega_odd, omega_even))
     return _collide(f, 27, coll, "collide_cumulant_geier_d3q27")
 
 

@@ -114,4 +114,4 @@ def test_cavity_layout_is_host_only_and_validates():
 def test_mlp_sizes_without_a_device():
     v = [ctypes.c_int64() for _ in range(4)]
     assert nat.lib().tlbm_mlp_sizes(*[ctypes.byref(x) for x in v]) == nat.OK
-    assert [x.value for x in v] == [1345, 1359, 4544, 64]
+    assert [x.value for x in v] == [1345, 1359, 3520, 64]

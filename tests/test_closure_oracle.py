@@ -186,3 +186,22 @@ def test_features_from_pi_use_the_reference_denominator():
     np.testing.assert_allclose(f[0, :6], [-p[0] / denom for p in pi], rtol=1e-14)
     s = f[0, :6]
     np.testing.assert_allclose(f[0, 6], np.sqrt(2 * (s[0] ** 2 + s[1] ** 2 + s[2] ** 2 + 2 * (s[3] ** 2 + s[4] ** 2 + s[5] ** 2))))
+
+
+def test_two_term_bf16_split_carries_16_bits():
+    rng = np.random.default_rng(14)
+    a = np.concatenate([rng.normal(0, 1, 4000), rng.normal(0, 1e-4, 4000)]).astype(np.float32)
+    hi, lo = oc.split_bf16(a)
+    rel = np.abs(hi + lo - a.astype(np.float64)) / np.abs(a.astype(np.float64))
+    assert rel.max() < 2.0**-15
+    assert np.abs(hi - a).max() > 0  # hi alone is bf16
+
+
+def test_forward_bf16_is_smooth_in_its_inputs():
+    """Inputs and activations carry ~16 bits, so a relative input nudge of 1e-6 moves ν by far less
+    than one bf16 step (2^-8 ≈ 4e-3 relative) would."""
+    vec = _params(15, 0.3)
+    x, _, mean, istd = _data(2000, 16)
+    y0 = oc.forward_bf16(vec, x, mean, istd)
+    y1 = oc.forward_bf16(vec, x * (1 + 1e-6), mean, istd)
+    assert np.max(np.abs(y1 - y0) / np.maximum(np.abs(y0), 1e-3)) < 2e-4

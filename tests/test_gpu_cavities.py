@@ -54,18 +54,37 @@ def test_batch_matches_oracle_fp64(dev):
                                    err_msg=f"cavity {i}")
 
 
-def test_batch_equals_single_grid_fused_step_bitwise(dev):
-    b, states = _batch(dev, torch.float32)
-    singles = []
+def _singles(dev, states, dtype):
+    out = []
     for i, e in enumerate(EDGES):
-        s = tl.FusedSolver3D(e, e, e, TAUS[i], dev, torch.float32, solid=wall_mask(e, dev), boundary="lid_y",
+        s = tl.FusedSolver3D(e, e, e, TAUS[i], dev, dtype, solid=wall_mask(e, dev), boundary="lid_y",
                              u_bc=(U_LID, 0.0, 0.0), layout="contiguous")
-        singles.append((s, torch.tensor(states[i], device=dev, dtype=torch.float32)))
+        out.append((s, torch.tensor(states[i], device=dev, dtype=dtype)))
+    return out
+
+
+def test_single_grid_lid_and_walls_match_oracle_fp64(dev):
+    """The fused step with the wall mask and TLBM_BC_LID_Y is the oracle's cavity chain."""
+    _, states = _batch(dev)
+    singles = _singles(dev, states, torch.float64)
+    for _ in range(3):
+        singles = [(s, s.step(f)) for s, f in singles]
+        states = [ref.cavity_step(x, t, U_LID) for x, t in zip(states, TAUS)]
+    for i, (_, f) in enumerate(singles):
+        np.testing.assert_allclose(f.cpu().numpy(), states[i], rtol=1e-12, atol=1e-14, err_msg=f"cavity {i}")
+
+
+def test_batch_matches_single_grid_fused_step_fp32(dev):
+    """Same arithmetic, different kernels: agreement to fp32 rounding (instruction selection may
+    contract multiply-adds differently)."""
+    b, states = _batch(dev, torch.float32)
+    singles = _singles(dev, states, torch.float32)
     for _ in range(4):
         b.step()
         singles = [(s, s.step(f)) for s, f in singles]
     for i, (_, f) in enumerate(singles):
-        assert torch.equal(b.populations(i), f), f"cavity {i}"
+        d = float((b.populations(i) - f).abs().max())
+        assert d < 2e-7, (i, d)
 
 
 @pytest.mark.parametrize("grid", [1, 7, 64, 4096])

@@ -47,7 +47,7 @@ def test_sizes_match_the_oracle_layout():
     s = cl.sizes()
     assert s["nparam"] == NP == 1345
     assert s["ntotal"] == NP + 14
-    assert s["packed_bytes"] == 4544
+    assert s["packed_bytes"] == 3520
 
 
 def test_predict_matches_oracle(dev):

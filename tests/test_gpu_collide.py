@@ -159,10 +159,12 @@ def test_smagorinsky_mrt(q, dtype, dev):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_geier_d3q27(dtype, dev):
-    f = _state(27)
-    got = tl.collide_cumulant_geier_d3q27(_gpu(f, dev, dtype), 0.58, omega_b=1.1, omega_odd=1.3, omega_even=0.9)
-    want = ref.collide_cumulant_geier_d3q27(f, 0.58, 1.1, 1.3, 0.9)
+@pytest.mark.parametrize("cs", [0.0, 0.2])
+def test_geier_d3q27(dtype, cs, dev):
+    f = _state(27, noise=3e-2)
+    got = tl.collide_cumulant_geier_d3q27(_gpu(f, dev, dtype), 0.58, omega_b=1.1, omega_odd=1.3, omega_even=0.9,
+                                          C_s=cs)
+    want = ref.collide_cumulant_geier_d3q27_les(f, 0.58, 1.1, 1.3, 0.9, C_s=cs)
     tol = dict(rtol=1e-4, atol=1e-6) if dtype == torch.float32 else TOL[dtype]
     g = got.cpu().numpy()
     assert np.allclose(g, want, **tol), float(np.abs(g - want).max())
