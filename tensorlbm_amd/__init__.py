@@ -21,6 +21,9 @@ from .fused import (
     hip_stream,
     is_available,
 )
+from .cavities import BatchedCavities
+from .closure import EddyViscosityMLP3D, TrainConfig3D, train_eddy_viscosity_model_3d
+from .executor import LBMStepExecutor
 from .ops import *  # noqa: F401,F403
 from .ops import __all__ as _ops_all
 from .precision import PrecisionPolicy, cast_to_compute, cast_to_store
@@ -40,5 +43,6 @@ __all__ = list(_ops_all) + [
     "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
     "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
     "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",
-    "TritonFusedSolver3D",
+    "TritonFusedSolver3D", "LBMStepExecutor", "BatchedCavities", "EddyViscosityMLP3D", "TrainConfig3D",
+    "train_eddy_viscosity_model_3d",
 ]
