@@ -75,8 +75,9 @@ class BatchedCavities:
         self.total_elements, self.total_tiles = tot.value, tiles.value
         self.edges = [int(e) for e in edges]
         self.taus = [float(t) for t in taus]
-        self.a = torch.empty(self.total_elements, dtype=dtype, device=self.device)
-        self.b = torch.empty_like(self.a)
+        # zero-filled: the plane padding is never written by the kernel
+        self.a = torch.zeros(self.total_elements, dtype=dtype, device=self.device)
+        self.b = torch.zeros_like(self.a)
         self.grid = 0
         self._graphs: dict = {}
 

@@ -40,6 +40,9 @@ __all__ = [
     "coarsen_moments",
     "sgs_samples",
     "les_samples_from_populations",
+    "les_tau",
+    "run_dns_reference_3d",
+    "les_initial_populations",
     "sizes",
 ]
 
@@ -323,6 +326,49 @@ def sgs_samples(fields: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
 def les_samples_from_populations(f: torch.Tensor, factor: int, *, deviation: bool = False):
     """``sgs_samples(coarsen_moments(f, factor))``."""
     return sgs_samples(coarsen_moments(f, factor, deviation=deviation))
+
+
+def les_tau(tau_dns: float, dns_scale: int) -> float:
+    """Relaxation time of the coarse grid under acoustic scaling (Δx, Δt both × r): ν_les = ν_dns / r."""
+    return 3.0 * ((float(tau_dns) - 0.5) / 3.0) / int(dns_scale) + 0.5
+
+
+def run_dns_reference_3d(n_les: int, tau_dns: float, n_steps_les: int, sample_every: int, seed: int,
+                         device: str | torch.device = "cuda:0", *, dns_scale: int = 3, k0: float = 4.0,
+                         u_rms: float = 0.03) -> list[torch.Tensor]:
+    """Coarse moment snapshots of a fine BGK run (the 3-D form of ``_run_dns_reference``,
+    ai/pipeline.py:110-149).
+
+    The fine grid is ``n_les · dns_scale`` per edge, started at rest density from the
+    random solenoidal field (benchmarks/pending/dit_turbulence/run.py:95-125, spectrum peak ``k0``,
+    ``u_rms``), advanced ``dns_scale`` fine steps per coarse step and block-averaged every
+    ``sample_every`` coarse steps.  Returns ``coarsen_moments`` fields, the first at t = 0.
+    """
+    from . import fields, ops
+    from .fused import FusedSolver3D
+
+    r = int(dns_scale)
+    n = int(n_les) * r
+    u = fields.random_solenoidal_field(n, k0, u_rms, seed=seed, device=device, dtype=torch.float32)
+    f = ops.equilibrium3d(torch.ones((n, n, n), device=device), u[0].contiguous(), u[1].contiguous(),
+                          u[2].contiguous())
+    del u
+    solver = FusedSolver3D(n, n, n, tau_dns, device, torch.float32, layout="contiguous")
+    snaps = [coarsen_moments(f, r)]
+    for k in range(1, int(n_steps_les) + 1):
+        for _ in range(r):
+            f = solver.step(f)
+        if sample_every > 0 and k % sample_every == 0:
+            snaps.append(coarsen_moments(f, r))
+    return snaps
+
+
+def les_initial_populations(fields: torch.Tensor) -> torch.Tensor:
+    """Equilibrium populations of a coarse snapshot's ρ̄ and ū (for starting an LES from filtered DNS)."""
+    from . import ops
+
+    return ops.equilibrium3d(fields[0].contiguous(), fields[1].contiguous(), fields[2].contiguous(),
+                             fields[3].contiguous())
 
 
 def clone_model(model: EddyViscosityMLP3D) -> EddyViscosityMLP3D:
