@@ -253,6 +253,8 @@ int tlbm_cavities_step(int store, const void* f_in, void* f_out, int64_t capacit
 typedef struct tlbm_graph tlbm_graph;
 int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const tlbm_cavity* cavs, int n,
                         double u_lid, int grid, int steps, tlbm_graph** graph);
+/* enqueue one replay on `stream`; on the null stream the replay runs on the graph's own stream,
+ * fenced by events so that it follows earlier and precedes later null-stream work */
 int tlbm_graph_launch(tlbm_graph* graph, tlbm_stream_t stream);
 int tlbm_graph_destroy(tlbm_graph* graph);
 

@@ -179,6 +179,8 @@ int launch(const CavArgs& a, int store, int grid, hipStream_t s) {
 struct tlbm_graph {
   hipGraph_t graph;
   hipGraphExec_t exec;
+  hipStream_t side;             // launch stream used when the caller passes the null stream
+  hipEvent_t before, after;     // orders the side stream against the null stream
 };
 
 extern "C" {
@@ -221,7 +223,7 @@ int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const
   hipStream_t s;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
     return cfail(TLBM_ERR_HIP, "hipStreamCreate failed");
-  tlbm_graph* g = new (std::nothrow) tlbm_graph{nullptr, nullptr};
+  tlbm_graph* g = new (std::nothrow) tlbm_graph{nullptr, nullptr, nullptr, nullptr, nullptr};
   if (!g) {
     (void)hipStreamDestroy(s);
     return cfail(TLBM_ERR_HIP, "out of host memory");
@@ -239,10 +241,12 @@ int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const
       rc = cfail(TLBM_ERR_HIP, "hipGraphInstantiate failed");
   }
   (void)hipStreamDestroy(s);
+  if (rc == TLBM_OK && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+                        hipEventCreateWithFlags(&g->before, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&g->after, hipEventDisableTiming) != hipSuccess))
+    rc = cfail(TLBM_ERR_HIP, "stream / event creation failed");
   if (rc != TLBM_OK) {
-    if (g->exec) (void)hipGraphExecDestroy(g->exec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
-    delete g;
+    tlbm_graph_destroy(g);
     return rc;
   }
   *out = g;
@@ -251,15 +255,31 @@ int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const
 
 int tlbm_graph_launch(tlbm_graph* g, tlbm_stream_t stream) {
   if (!g || !g->exec) return cfail(TLBM_ERR_INVALID, "null graph");
-  const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  if (s != nullptr) {
+    const hipError_t e = hipGraphLaunch(g->exec, s);
+    if (e != hipSuccess) return cfail(TLBM_ERR_HIP, "hipGraphLaunch failed (%lld)", int(e));
+    return TLBM_OK;
+  }
+  // null stream: run the graph on a private stream fenced by events on both sides,
+  // so it starts after and finishes before the null stream's neighbouring work
+  if (hipEventRecord(g->before, nullptr) != hipSuccess || hipStreamWaitEvent(g->side, g->before, 0) != hipSuccess)
+    return cfail(TLBM_ERR_HIP, "event fence before the graph failed");
+  const hipError_t e = hipGraphLaunch(g->exec, g->side);
   if (e != hipSuccess) return cfail(TLBM_ERR_HIP, "hipGraphLaunch failed (%lld)", int(e));
+  if (hipEventRecord(g->after, g->side) != hipSuccess || hipStreamWaitEvent(nullptr, g->after, 0) != hipSuccess)
+    return cfail(TLBM_ERR_HIP, "event fence after the graph failed");
   return TLBM_OK;
 }
 
 int tlbm_graph_destroy(tlbm_graph* g) {
   if (!g) return TLBM_OK;
+  if (g->side) (void)hipStreamSynchronize(g->side);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->before) (void)hipEventDestroy(g->before);
+  if (g->after) (void)hipEventDestroy(g->after);
+  if (g->side) (void)hipStreamDestroy(g->side);
   delete g;
   return TLBM_OK;
 }

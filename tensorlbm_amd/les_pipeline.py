@@ -1,0 +1,100 @@
+"""DNS → learned closure → LES, end to end on one GPU (the 3-D ``run_ai_les_pipeline``).
+
+The reference's pipeline (src/tensorlbm/ai/pipeline.py:222-398) runs a finer BGK reference,
+block-averages it, trains ``EddyViscosityMLP`` and validates an LES that uses it.  Here:
+
+1. ``run_dns_reference_3d``: fine D3Q19 BGK from the random solenoidal field, filtered by
+   ``dns_scale``³ blocks every ``sample_every`` coarse steps (training run, one seed);
+2. ``sgs_samples`` + ``train_eddy_viscosity_model_3d`` on the GPU kernels;
+3. a second DNS (another seed) is the truth; LES runs of the same filtered initial state —
+   no closure, Smagorinsky, learned closure — in the storage given by ``les_dtype``
+   (``float16`` uses the deviation form), closure evaluated inside the fused step;
+4. per snapshot: velocity rel-L2 and kinetic-energy error against the filtered DNS.
+
+``python -m tensorlbm_amd.les_pipeline --n-les 192 --dns-scale 3`` is the 576³ → 192³ case
+(about 29 GB for the DNS); smaller sizes run in seconds.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import torch
+
+from . import closure as cl
+from .fused import FusedSolver3D
+from .lattice import D3Q19
+from .ops import macroscopic3d
+
+__all__ = ["run_ai_les_pipeline_3d"]
+
+
+def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32, device="cuda:0"):
+    n = int(test[0].shape[1])
+    f = cl.les_initial_populations(test[0])
+    dev_store = dtype in (torch.float16, torch.bfloat16)
+    if dev_store:
+        f = (f - D3Q19.W_EXACT64.to(f.device, f.dtype).reshape(19, 1, 1, 1)).to(dtype)
+    s = FusedSolver3D(n, n, n, tau, device, dtype, deviation=dev_store, closure=closure, C_s=C_s)
+    errs, kes = [], []
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        f = s.step(f)
+        if k % every == 0:
+            g = f.float()
+            if dev_store:
+                g = g + D3Q19.W_EXACT64.to(g.device, g.dtype).reshape(19, 1, 1, 1)
+            _, ux, uy, uz = macroscopic3d(g.contiguous())
+            u = torch.stack([ux, uy, uz]).double()
+            ref = test[k // every][1:4].double()
+            errs.append(float(torch.linalg.vector_norm(u - ref) / torch.linalg.vector_norm(ref)))
+            kes.append(float((u * u).sum() / (ref * ref).sum()) - 1.0)
+    torch.cuda.synchronize()
+    return {"err_final": errs[-1], "err_mean": sum(errs) / len(errs), "ke_err_final": kes[-1],
+            "ke_err_mean_abs": sum(abs(v) for v in kes) / len(kes), "wall_s": time.perf_counter() - t0}
+
+
+def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float = 0.51, steps: int = 300,
+                           sample_every: int = 10, train_seed: int = 101, test_seed: int = 202, epochs: int = 30,
+                           C_s: float = 0.17, les_dtype: torch.dtype = torch.float32,
+                           device: str | torch.device = "cuda:0") -> dict:
+    t0 = time.perf_counter()
+    train = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, train_seed, device, dns_scale=dns_scale)
+    x, t = zip(*(cl.sgs_samples(s) for s in train[1:]))
+    del train
+    feats, labels = torch.cat(x), torch.cat(t)
+    del x, t
+    t_dns = time.perf_counter() - t0
+    model, meta = cl.train_eddy_viscosity_model_3d(feats, labels, cl.TrainConfig3D(epochs=epochs), device=device)
+    del feats, labels
+    test = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, test_seed, device, dns_scale=dns_scale)
+    tau = cl.les_tau(tau_dns, dns_scale)
+    runs = {
+        "no_closure": _les(test, tau, steps, sample_every, dtype=les_dtype, device=device),
+        "smagorinsky": _les(test, tau, steps, sample_every, C_s=C_s, dtype=les_dtype, device=device),
+        "learned": _les(test, tau, steps, sample_every, closure=model, C_s=C_s, dtype=les_dtype, device=device),
+    }
+    return {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
+            "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
+            "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--n-les", type=int, default=32)
+    p.add_argument("--dns-scale", type=int, default=3)
+    p.add_argument("--tau-dns", type=float, default=0.51)
+    p.add_argument("--steps", type=int, default=300)
+    p.add_argument("--every", type=int, default=10)
+    p.add_argument("--epochs", type=int, default=30)
+    p.add_argument("--les-dtype", choices=("float32", "float16"), default="float32")
+    a = p.parse_args()
+    res = run_ai_les_pipeline_3d(a.n_les, a.dns_scale, a.tau_dns, a.steps, a.every, epochs=a.epochs,
+                                 les_dtype=getattr(torch, a.les_dtype))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

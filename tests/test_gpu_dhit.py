@@ -1,0 +1,62 @@
+"""Decaying homogeneous isotropic turbulence: reduced-precision storage against FP64, per field."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import tensorlbm_amd as tl
+from tensorlbm_amd import diagnostics as dg
+from tensorlbm_amd import fields
+from tensorlbm_amd.closure import EddyViscosityMLP3D
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS = 48, 150
+
+
+def _init(lat, dtype, deviation, dev):
+    u = fields.random_solenoidal_field(N, 4.0, 0.03, seed=7, device=dev, dtype=torch.float64)
+    rho = torch.ones((N, N, N), device=dev, dtype=torch.float64)
+    eq = tl.equilibrium27 if lat == "D3Q27" else tl.equilibrium3d
+    f = eq(rho, u[0].contiguous(), u[1].contiguous(), u[2].contiguous())
+    if deviation:
+        q = f.shape[0]
+        f = f - tl.lattice.BY_Q[q].W_EXACT64.to(dev).reshape(q, 1, 1, 1)
+    return f.to(dtype)
+
+
+def _run(lat, coll, tau, dtype, deviation, dev, closure=None):
+    f = _init(lat, dtype, deviation, dev)
+    s = tl.FusedSolver3D(N, N, N, tau, dev, dtype, lattice=lat, collision=coll, deviation=deviation,
+                         closure=closure, C_s=0.17 if closure is not None else 0.0)
+    for _ in range(STEPS):
+        f = s.step(f)
+    return f
+
+
+@pytest.mark.parametrize("lat,coll,tau", [("D3Q27", "GEIER", 0.505), ("D3Q19", "BGK", 0.51)])
+def test_deviation_storage_errors_vs_fp64(dev, lat, coll, tau):
+    ref = _run(lat, coll, tau, torch.float64, False, dev)
+    e32 = dg.field_errors(_run(lat, coll, tau, torch.float32, False, dev), ref)
+    e16 = dg.field_errors(_run(lat, coll, tau, torch.float16, True, dev), ref, deviation=True)
+    eb16 = dg.field_errors(_run(lat, coll, tau, torch.bfloat16, True, dev), ref, deviation=True)
+    print(f"{lat} {coll} tau={tau}: fp32 {e32}\n fp16-dev {e16}\n bf16-dev {eb16}")
+    assert e32["u"] < 5e-5 and e32["pi"] < 5e-4
+    assert e16["u"] < 1e-3 and e16["pi"] < 5e-3
+    assert eb16["u"] < 1e-2 and eb16["pi"] < 5e-2
+    assert e32["u"] < e16["u"] < eb16["u"]
+
+
+def test_fp16_storage_with_learned_closure_vs_fp64(dev):
+    """The FP16-store + closure form stays within 1e-3 (velocity, relative L2) of its FP64 run."""
+    m = EddyViscosityMLP3D(dev, seed=3)
+    m.set_feature_stats(torch.zeros(7), torch.full((7,), 1e-3))
+    ref = _run("D3Q19", "BGK", 0.51, torch.float64, False, dev, closure=m)
+    e32 = dg.field_errors(_run("D3Q19", "BGK", 0.51, torch.float32, False, dev, closure=m), ref)
+    e16 = dg.field_errors(_run("D3Q19", "BGK", 0.51, torch.float16, True, dev, closure=m), ref, deviation=True)
+    print(f"closure: fp32 {e32}\n fp16-dev {e16}")
+    # the closure's inputs carry ~16 bits: precision changes do not flip its output
+    assert e32["u"] < 1e-4
+    assert e16["u"] < 1e-3
