@@ -132,6 +132,31 @@ def measured_traffic(n: int):
     return None
 
 
+def dhit_forms(torch, dev, n, steps=100):
+    """Decaying isotropic turbulence, D3Q27 Geier cumulants, τ = 0.505, random solenoidal start
+    (k0 = 4, u_rms = 0.03): throughput of fp32 and of the fp16 / bf16 deviation storages."""
+    import tensorlbm_amd as tl
+    from tensorlbm_amd import fields
+
+    u = fields.random_solenoidal_field(n, 4.0, 0.03, seed=SEED, device=dev, dtype=torch.float32)
+    f0 = tl.equilibrium27(torch.ones((n, n, n), device=dev), u[0].contiguous(), u[1].contiguous(), u[2].contiguous())
+    del u
+    w = tl.lattice.D3Q27.W_EXACT64.to(dev).view(27, 1, 1, 1)
+    out = {}
+    for name, dtype, dv in (("fp32", torch.float32, False), ("fp16dev", torch.float16, True),
+                            ("bf16dev", torch.bfloat16, True)):
+        f = (f0 - w).to(dtype) if dv else f0.clone()
+        s = tl.FusedSolver3D(n, n, n, 0.505, dev, dtype, lattice="D3Q27", collision="GEIER", deviation=dv)
+        ms, _, f = time_steps(s.step, f, steps, 10, torch)
+        bpc = 2 * 27 * torch.tensor([], dtype=dtype).element_size()
+        out[name] = {"ms_per_step": round(ms, 5), "mlups": round(n**3 / (ms * 1e-3) / 1e6, 1), "bytes_per_cell": bpc,
+                     "hbm_frac": round(bpc * n**3 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del s, f
+    del f0
+    torch.cuda.empty_cache()
+    return out
+
+
 def cavities_form(torch, dev, steps=100):
     """64 lid-driven cavities (edges U{32..160}, Re log-uniform [100, 1000]): one autotuned launch per step
     replayed from a HIP graph, against 64 single-cavity fused launches per step."""
@@ -230,6 +255,7 @@ def main():
         forms["bgk_fp32_mlp_channel_sphere"] = form(torch.float32, mlp=True)
         forms["fp16dev_mlp_speedup_vs_fp32"] = round(forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"] / head["mlups"], 3)
         forms["cavities_64"] = cavities_form(torch, dev)
+        forms["dhit_d3q27_geier"] = dhit_forms(torch, dev, n)
     if args.copy_probe:
         src = f0.clone()
         dst = torch.empty_like(src)
