@@ -113,6 +113,31 @@ __device__ __forceinline__ void neq_stress(const C (&fneq)[L::Q], C (&pi)[6]) {
   }
 }
 
+// Pi^neq = sum_q c_a c_b f_q - rho (delta_ab / 3 + u_a u_b): the second moments of
+// the second-order equilibrium are exact on lattices with 4th-order isotropic
+// weights (D2Q9, D3Q19, D3Q27), so this equals neq_stress(f - f_eq) without
+// forming f_eq.
+template <class L, typename C>
+__device__ __forceinline__ void neq_stress_moments(const C (&f)[L::Q], C rho, C ux, C uy, C uz, C (&pi)[6]) {
+  C m[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    if (L::cx[q] != 0) m[0] += f[q];
+    if (L::cy[q] != 0) m[1] += f[q];
+    if (L::cz[q] != 0) m[2] += f[q];
+    if (L::cx[q] * L::cy[q] != 0) m[3] += C(L::cx[q] * L::cy[q]) * f[q];
+    if (L::cx[q] * L::cz[q] != 0) m[4] += C(L::cx[q] * L::cz[q]) * f[q];
+    if (L::cy[q] * L::cz[q] != 0) m[5] += C(L::cy[q] * L::cz[q]) * f[q];
+  }
+  const C r3 = rho * C(1.0 / 3.0);
+  pi[0] = m[0] - (r3 + rho * ux * ux);
+  pi[1] = m[1] - (r3 + rho * uy * uy);
+  pi[2] = m[2] - (r3 + rho * uz * uz);
+  pi[3] = m[3] - rho * ux * uy;
+  pi[4] = m[4] - rho * ux * uz;
+  pi[5] = m[5] - rho * uy * uz;
+}
+
 template <typename C>
 __device__ __forceinline__ C pi_norm(const C (&pi)[6]) {
   return sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +

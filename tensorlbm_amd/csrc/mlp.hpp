@@ -17,7 +17,9 @@
 // block (one cross-half lane exchange supplies the partner lane's half), the
 // layer-1 C tile is reused as the layer-2 B operand in registers (W2's columns
 // are permuted at packing time to match), and the output layer is 16 local
-// FMAs plus one cross-half shuffle.  Ten matrix-core tiles per wave.
+// FMAs plus one cross-half shuffle.  Ten matrix-core tiles per wave.  The
+// operand blob (3.5 KB) is read straight from global memory, where it stays
+// cache-resident across the launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -117,7 +119,7 @@ __device__ __forceinline__ float block_out(const Packed* P, int h, f32x16 c, bf1
 
 // Eddy viscosity of the calling lane's cell.  Must be called by all 64 lanes
 // of the wave (inactive cells pass any finite features and ignore the result).
-// P points to the packed blob (LDS or global), x to 7 raw features.
+// P points to the packed blob, x to 7 raw features.
 __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
   const int lane = __lane_id();
   const int h = lane >> 5;

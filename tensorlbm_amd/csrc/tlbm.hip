@@ -108,15 +108,6 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
   constexpr bool kMlp = (FEAT & FEAT_MLP) != 0;
 
-  if constexpr (kMlp) {
-    // stage the matrix-core operand blob in LDS (4544 B = 284 x 16 B)
-    const uint4* src = static_cast<const uint4*>(a.mlp);
-    uint4* dst = reinterpret_cast<uint4*>(dyn_lds);
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    for (int i = tid; i < int(sizeof(mlp::Packed) / 16); i += 256) dst[i] = src[i];
-    __syncthreads();
-  }
-
   const uint32_t tile = blockIdx.x;
   const uint32_t txi = tile % uint32_t(a.tiles_x);
   const uint32_t rest = tile / uint32_t(a.tiles_x);
@@ -214,11 +205,8 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
     if constexpr (COLL == TLBM_COLL_BGK && kMlp) {
       // learned closure: strain from the non-equilibrium stress with the
       // Smagorinsky tau as reference time scale, S = -Pi / (2 rho cs^2 tau_ref)
-      C feq[Q], fneq[Q], pi[6];
-      equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
-      neq_stress<L>(fneq, pi);
+      C pi[6];
+      neq_stress_moments<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
       const C om_ref = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
       const C sc = C(-1.5) * om_ref / mc.rho_safe;
       float xs[7];
@@ -226,7 +214,8 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
       for (int k = 0; k < 6; ++k) xs[k] = float(sc * pi[k]);
       xs[6] = sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
                             2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
-      const float nu = mlp::nu_t(reinterpret_cast<const mlp::Packed*>(dyn_lds), xs);
+      // operands straight from the (cache-resident, 3.5 KB) blob: no staging barrier
+      const float nu = mlp::nu_t(static_cast<const mlp::Packed*>(a.mlp), xs);
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
@@ -647,7 +636,7 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
 template <class L, int S, int COLL, bool STREAM, int FEAT>
 int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
   if constexpr ((FEAT & FEAT_MLP) != 0) {
-    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), sizeof(mlp::Packed), s, a);
+    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
   }
