@@ -70,7 +70,8 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
 #define TLBM_BC_NONE 0
 #define TLBM_BC_CHANNEL_X 1 /* equilibrium inlet x=0 (rho from x=1), copy outlet x=nx-1,
                                boundaries3d.py:407-443 */
-#define TLBM_BC_LID_Y 2     /* Zou-He moving lid on y=ny-1, boundaries3d.py:324-404 */
+#define TLBM_BC_LID_Y 2     /* Zou-He moving lid on the fluid cells of y=ny-1 (solid cells there
+                               keep bounce-back), boundaries3d.py:324-404 */
 #define TLBM_BC_FAR_FIELD 3 /* free-stream equilibrium (rho = 1, u) on the inlet x=0 and the lateral
                                faces named in `faces`, copy outlet x=nx-1 (far_field_bc_3d,
                                boundaries3d.py:446-537; lateral faces win on shared edges) */
@@ -219,8 +220,8 @@ int tlbm_sgs_samples(const float* fields, int nz, int ny, int nx, float* feats, 
 /*
  * Batched lid-driven cavities (benchmark case of many independent cubic D3Q19 BGK cavities).
  * Each cavity of edge L is the reference's cavity step chain on an L^3 grid: pull stream,
- * Zou-He moving lid on y = L-1 (zou_he_moving_lid_3d, boundaries3d.py:324-404, every cell of
- * the plane), full-way bounce-back walls x = 0, x = L-1, y = 0, z = 0, z = L-1
+ * Zou-He moving lid on the fluid cells of y = L-1 (zou_he_moving_lid_3d, boundaries3d.py:324-404),
+ * full-way bounce-back walls x = 0, x = L-1, y = 0, z = 0, z = L-1
  * (bounce_back_cells_3d, boundaries3d.py:112-132; walls of cases/cavity.py:72-79 closed in z),
  * then BGK (collide_bgk3d, solver3d.py:100-104) -- per cavity the same result as tlbm_step
  * with that wall mask and TLBM_BC_LID_Y.  No reference entry point runs a batch in one launch.

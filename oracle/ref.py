@@ -849,10 +849,12 @@ def cavity_walls(edge: int) -> np.ndarray:
 
 
 def cavity_step(f: np.ndarray, tau: float, u_lid: float) -> np.ndarray:
-    """Pull stream, Zou-He lid on y = L−1 (every cell of the plane, boundaries3d.py:324-404),
-    full-way bounce-back on the walls, BGK with the fused kernels' floored-density equilibrium."""
-    fin = zou_he_moving_lid(stream(f), u_lid)
-    fin = bounce_back(fin, cavity_walls(f.shape[1]))
+    """Pull stream, Zou-He lid on the fluid cells of y = L−1 (boundaries3d.py:324-404), full-way
+    bounce-back on the walls, BGK with the fused kernels' floored-density equilibrium."""
+    walls = cavity_walls(f.shape[1])
+    fs = stream(f)
+    fin = np.where(walls[None], fs, zou_he_moving_lid(fs, u_lid))
+    fin = bounce_back(fin, walls)
     feq, _, _ = _feq_safe(fin)
     return fin - (1.0 / tau) * (fin - feq)
 

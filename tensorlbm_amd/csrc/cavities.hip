@@ -3,9 +3,11 @@
 //
 // Each cavity of edge L is the reference's cavity step chain on an L^3 grid
 // (full-way bounce-back walls on x = 0, x = L-1, y = 0, z = 0, z = L-1 and the
-// Zou-He moving lid of boundaries3d.py:324-404 on y = L-1, every cell of that
-// plane; pull stream, then lid, then wall swap, then BGK), i.e. exactly what
-// tlbm_step computes for that cavity alone with the wall mask and TLBM_BC_LID_Y.
+// Zou-He moving lid of boundaries3d.py:324-404 on the fluid cells of y = L-1;
+// pull stream, then lid, then wall swap, then BGK), i.e. exactly what tlbm_step
+// computes for that cavity alone with the wall mask and TLBM_BC_LID_Y.  (Zou-He
+// on the wall cells of the lid plane, whose values only bounce back, makes this
+// scheme diverge within ~100 steps at u_lid = 0.1.)
 // Walls are decided from (x, y, z) in registers: no mask is read.
 //
 // Memory: one buffer holds every cavity's populations, cavity i at element
@@ -94,8 +96,9 @@ __global__ __launch_bounds__(CAV_TILE) void k_cavities(const CavArgs a) {
       const T* base = fin + int64_t(q) * plane + int64_t(sz) * zs;
       f[q] = ST::template get<L>(base[uint32_t(sy) * uint32_t(E) + uint32_t(sx)], q);
     }
-    if (y == E - 1) zou_he_lid(f, C(a.u_lid));
-    if (x == 0 || x == E - 1 || y == 0 || z == 0 || z == E - 1) {
+    const bool wall = x == 0 || x == E - 1 || y == 0 || z == 0 || z == E - 1;
+    if (y == E - 1 && !wall) zou_he_lid(f, C(a.u_lid));
+    if (wall) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         if (q < L::opp[q]) {

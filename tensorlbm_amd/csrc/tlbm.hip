@@ -168,7 +168,8 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = pull_one<ST, L>(fin, a, q, nx - 2, yy, z);
       }
-    } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1) {
+    } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1 && !solid) {
+      // Zou-He on the fluid cells of the lid plane; solid cells there keep full-way bounce-back
       if constexpr (Q == 19) zou_he_lid(f, a.bc_u[0]);
     } else if (STREAM && a.bc_kind == TLBM_BC_FAR_FIELD) {
       // far_field_bc_3d (boundaries3d.py:446-537): free-stream equilibrium on the

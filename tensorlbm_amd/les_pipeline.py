@@ -59,23 +59,29 @@ def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32,
 def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float = 0.51, steps: int = 300,
                            sample_every: int = 10, train_seed: int = 101, test_seed: int = 202, epochs: int = 30,
                            C_s: float = 0.17, les_dtype: torch.dtype = torch.float32,
-                           device: str | torch.device = "cuda:0") -> dict:
+                           device: str | torch.device = "cuda:0", log=None) -> dict:
+    def say(msg):
+        if log is not None:
+            log(f"[les_pipeline {time.perf_counter() - t0:7.1f} s] {msg}")
+
     t0 = time.perf_counter()
     train = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, train_seed, device, dns_scale=dns_scale)
     x, t = zip(*(cl.sgs_samples(s) for s in train[1:]))
     del train
     feats, labels = torch.cat(x), torch.cat(t)
+    say(f"training DNS done: {feats.shape[0]} samples")
     del x, t
     t_dns = time.perf_counter() - t0
     model, meta = cl.train_eddy_viscosity_model_3d(feats, labels, cl.TrainConfig3D(epochs=epochs), device=device)
     del feats, labels
+    say(f"trained: val R2 {meta['final_val_r2']:.3f} in {meta['training_time_s']:.1f} s")
     test = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, test_seed, device, dns_scale=dns_scale)
     tau = cl.les_tau(tau_dns, dns_scale)
-    runs = {
-        "no_closure": _les(test, tau, steps, sample_every, dtype=les_dtype, device=device),
-        "smagorinsky": _les(test, tau, steps, sample_every, C_s=C_s, dtype=les_dtype, device=device),
-        "learned": _les(test, tau, steps, sample_every, closure=model, C_s=C_s, dtype=les_dtype, device=device),
-    }
+    say("test DNS done")
+    runs = {}
+    for name, kw in (("no_closure", {}), ("smagorinsky", {"C_s": C_s}), ("learned", {"closure": model, "C_s": C_s})):
+        runs[name] = _les(test, tau, steps, sample_every, dtype=les_dtype, device=device, **kw)
+        say(f"LES {name}: {runs[name]}")
     return {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
             "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
             "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
@@ -92,7 +98,7 @@ def main():
     p.add_argument("--les-dtype", choices=("float32", "float16"), default="float32")
     a = p.parse_args()
     res = run_ai_les_pipeline_3d(a.n_les, a.dns_scale, a.tau_dns, a.steps, a.every, epochs=a.epochs,
-                                 les_dtype=getattr(torch, a.les_dtype))
+                                 les_dtype=getattr(torch, a.les_dtype), log=lambda m: print(m, flush=True))
     print(json.dumps(res), flush=True)
 
 
