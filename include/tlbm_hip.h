@@ -131,6 +131,23 @@ int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_stride
               const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,
               const int8_t* solid, const tlbm_boundary* bc, double* force, tlbm_stream_t stream);
 
+/*
+ * Two consecutive tlbm_step()s in one launch, f_in -> f_out (temporal blocking: the
+ * intermediate lattice stays in LDS).  Replaces two iterations of the reporter-free
+ * run() loop, src/tensorlbm/triton_fused.py:892-895.  Same arguments and results as
+ * tlbm_step(f_in -> tmp) then tlbm_step(tmp -> f_out); force receives the sum of
+ * both steps' samples.  D3Q19 / D3Q27; F32, F16_DEV, BF16_DEV storage; every
+ * collision and boundary except TAU_MLP.  Returns TLBM_ERR_UNSUPPORTED for other
+ * arguments, which tlbm_step2_supported() reports without launching.
+ */
+int tlbm_step2(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+               const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,
+               const int8_t* solid, const tlbm_boundary* bc, double* force, tlbm_stream_t stream);
+
+/* 1 if tlbm_step2() serves these arguments, else 0; sets no error */
+int tlbm_step2_supported(int lattice, int store, const tlbm_collision* coll, const tlbm_boundary* bc,
+                         int nx);
+
 /* periodic pull stream (solver3d.py:210-239 stream3d, solver.py:396-417, d3q27.py:766-791) */
 int tlbm_stream(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
                 const int64_t* out_strides, int nz, int ny, int nx, tlbm_stream_t stream);
@@ -188,7 +205,7 @@ int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, 
 /* sizes of the parameter vector, the packed matrix-core blob and the gradient block */
 int tlbm_mlp_sizes(int64_t* nparam, int64_t* ntotal, int64_t* packed_bytes, int64_t* batch_block);
 
-/* fp32 parameters -> packed bf16 blob used by tlbm_step with TLBM_TAU_MLP (coll->mlp) */
+/* fp32 parameters -> packed fp16 matrix-core blob used by tlbm_step with TLBM_TAU_MLP (coll->mlp) */
 int tlbm_mlp_pack(const float* params, void* packed, tlbm_stream_t stream);
 
 /* y[n] = MLP(x[n][7]) in fp32 (predict_tau_eff_2d's network call, ai/inference.py:53-65) */

@@ -16,7 +16,7 @@
 
 Pinned in ``tests/test_closure_oracle.py``: gradients by torch autograd and
 finite differences, Adam + clipping by ``torch.optim.Adam`` and
-``clip_grad_norm_``, ``bf16`` by torch's bfloat16 cast, ``coarsen`` / ``strain``
+``clip_grad_norm_``, ``fp16`` by torch's half cast, ``coarsen`` / ``strain``
 / ``sgs_samples`` by constructed fields with closed-form answers, and the
 initial field by divergence / spectrum-peak / u_rms properties.
 """
@@ -92,33 +92,32 @@ def clip_grad_norm(grad, max_norm):
     return grad * min(1.0, max_norm / (norm + 1e-6))
 
 
-def bf16(a):
-    """Round to bfloat16 (nearest, ties to even) and return as float64."""
-    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+def fp16(a):
+    """Round fp32 values to binary16 (nearest, ties to even; overflow to ±inf) and return as float64."""
+    return np.asarray(a, np.float32).astype(np.float16).astype(np.float64)
 
 
-def split_bf16(a):
-    """Two-term bf16 split a ≈ hi + lo, hi = bf16(a), lo = bf16(a − hi), of fp32 values."""
+def split_fp16(a):
+    """Two-term fp16 split a ≈ hi + lo, hi = fp16(a), lo = fp16(a − hi), of fp32 values."""
     a32 = np.asarray(a, np.float32)
-    hi = bf16(a32)
-    lo = bf16((a32.astype(np.float64) - hi).astype(np.float32))
+    hi = fp16(a32)
+    lo = fp16((a32.astype(np.float64) - hi).astype(np.float32))
     return hi, lo
 
 
-def forward_bf16(vec, x, mean, inv_std):
-    """ν_t as the fused collision evaluates it on the matrix cores: bf16 W1 and W2, normalised
-    inputs and layer-1 activations as two-term bf16 splits, wide accumulation, fp32 output layer."""
+def forward_mfma(vec, x, mean, inv_std):
+    """ν_t as the fused collision evaluates it on the matrix cores: fp16 W1 and W2, normalised
+    inputs as two-term fp16 splits, layer-1 activations relu(z1) rounded once to fp16, wide
+    accumulation, fp32 output layer."""
     p = unpack(np.asarray(vec, dtype=np.float64))
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
     xh = f32((f32(x) - f32(mean)) * f32(inv_std))
-    xhi, xlo = split_bf16(xh)
-    w1 = bf16(p["w1"])
-    z1 = xhi @ w1.T + xlo @ w1.T + f32(p["b1"])
-    hhi, hlo = split_bf16(np.maximum(z1, 0.0))
-    w2 = bf16(p["w2"])
-    z2 = hhi @ w2.T + hlo @ w2.T + f32(p["b2"])
+    xhi, xlo = split_fp16(xh)
+    w1 = fp16(p["w1"])
+    z1 = f32(xhi @ w1.T + xlo @ w1.T + f32(p["b1"]))
+    h1 = fp16(np.maximum(z1, 0.0))
+    w2 = fp16(p["w2"])
+    z2 = f32(h1 @ w2.T + f32(p["b2"]))
     h2 = np.maximum(z2, 0.0)
     z3 = h2 @ f32(p["w3"]) + f32(p["b3"])[0]
     return softplus(z3)
@@ -240,7 +239,7 @@ def shell_spectrum(u):
 # ---------------------------------------------------------------------------
 
 
-def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_bf16=True):
+def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_mfma=True):
     """Periodic pull-stream then BGK with τ_eff = max(τ + 3 ν_t, 0.501) (ai/inference.py:56-92).
 
     ν_t = MLP(S, |S|) with S = −Π^neq / (2 ρ c_s² τ_ref) (turbulence.py:1449-1480), τ_ref the
@@ -257,7 +256,7 @@ def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_bf16=True):
     pi = ref.neq_stress(fneq)
     te_ref = ref.smagorinsky_tau(tau, ref.pi_norm(fneq), rho_safe, cs_delta_sq)
     x = features_from_pi(pi, rho_safe, te_ref).reshape(-1, N_IN)
-    nu = forward_bf16(vec, x, mean, istd) if emulate_bf16 else forward(vec, x, mean, istd)[0]
+    nu = forward_mfma(vec, x, mean, istd) if emulate_mfma else forward(vec, x, mean, istd)[0]
     nu = nu.reshape(rho_safe.shape)
     te = np.maximum(tau + 3.0 * nu, 0.501)
     return fin - fneq / te[None], nu

@@ -99,6 +99,9 @@ _SIGNATURES = {
     "tlbm_status_string": ([_I], ctypes.c_char_p),
     "tlbm_step": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP,
                    ctypes.POINTER(Boundary), _VP, _VP], _I),
+    "tlbm_step2": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP,
+                    ctypes.POINTER(Boundary), _VP, _VP], _I),
+    "tlbm_step2_supported": ([_I, _I, ctypes.POINTER(Collision), ctypes.POINTER(Boundary), _I], _I),
     "tlbm_stream": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, _VP], _I),
     "tlbm_collide": ([_I, _I, _VP, _VP, _VP, _VP, _I, _I, _I, ctypes.POINTER(Collision), _VP], _I),
     "tlbm_equilibrium": ([_I, _I, _VP, _VP, _VP, _VP, _VP, _I64, _VP], _I),

@@ -10,7 +10,7 @@ clipping, and ``collide_ai_les_bgk`` (ai/inference.py:68-92) relaxes with
   training set's mean / std (``set_feature_stats``, model.py:66-80);
 * inference runs **inside** the fused collision (``FusedSolver3D(closure=model)``): the strain
   comes from the non-equilibrium stress in registers and the two hidden layers run on the
-  matrix cores (bf16 operands, fp32 accumulation), so the closure adds no HBM traffic;
+  matrix cores (fp16 operands, fp32 accumulation), so the closure adds no HBM traffic;
 * training runs on the GPU through the library's own forward / backward / Adam kernels
   (``tlbm_mlp_grad`` + ``tlbm_mlp_adam``), deterministic for a given seed;
 * training data come from a DNS: block-averaged moments (``coarsen_moments``, the 3-D
@@ -117,7 +117,7 @@ class EddyViscosityMLP3D:
 
     @property
     def packed(self) -> torch.Tensor:
-        """Matrix-core operand blob (bf16 weights) for the fused collision, rebuilt when stale."""
+        """Matrix-core operand blob (fp16 weights) for the fused collision, rebuilt when stale."""
         if self._packed_version != self._version:
             nat.check(nat.lib().tlbm_mlp_pack(self.params.data_ptr(), self._packed.data_ptr(),
                                               nat.stream_of(self.params)), "tlbm_mlp_pack")

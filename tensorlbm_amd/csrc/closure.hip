@@ -30,12 +30,15 @@ constexpr int GRAD_BS = 64;  // samples per workgroup in the gradient kernel
 // packing: fp32 parameters -> matrix-core operand blob
 // ---------------------------------------------------------------------------
 
+// fp16 bits of v, round to nearest even
+__device__ __forceinline__ uint16_t f16_bits(float v) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v)); }
+
 __global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ p, Packed* __restrict__ out) {
   const int l = threadIdx.x;
   const int h = l >> 5, i = l & 31;
   for (int e = 0; e < 8; ++e) {
-    out->a1[l][e] = e < NIN ? bf16_bits(p[OW1 + i * NIN + e]) : uint16_t(0);
-    for (int s = 0; s < 2; ++s) out->a2[s][l][e] = bf16_bits(p[OW2 + i * NH + w2_col(s, h, e)]);
+    out->a1[l][e] = e < NIN ? f16_bits(p[OW1 + i * NIN + e]) : uint16_t(0);
+    for (int s = 0; s < 2; ++s) out->a2[s][l][e] = f16_bits(p[OW2 + i * NH + w2_col(s, h, e)]);
   }
   if (l < 32) {
     const int hh = l >> 4, r = l & 15;

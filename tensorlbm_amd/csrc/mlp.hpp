@@ -3,23 +3,21 @@
 // EddyViscosityMLP (src/tensorlbm/ai/model.py:40-92).
 //
 // Inference inside the collision runs on the matrix cores: per wave (64 cells,
-// one per lane) the two hidden layers are v_mfma_f32_32x32x16_bf16 tiles
-// (bf16 operands, fp32 accumulation).  Layout (wave64, 32x32x16):
+// one per lane) the two hidden layers are v_mfma_f32_32x32x16_f16 tiles (fp16
+// operands, exact products, fp32 accumulation).  Layout (wave64, 32x32x16):
 //   A[i][k] : lane i + 32*(k/8), element k%8
 //   B[k][j] : lane j + 32*(k/8), element k%8
 //   C[i][j] : lane j + 32*((i/4)%2), element (i%4) + 4*(i/8)
-// Hidden units are rows (i) and cells are columns (j).  Inputs and layer-1
-// activations enter as two-term bf16 splits v = hi + lo (hi = bf16(v),
-// lo = bf16(v - hi)), so the network sees them to ~2^-16 relative instead of
-// bf16's 2^-8: the closure stays a smooth function of the flow and runs in
-// different precisions do not drift apart through rounding flips.  Layer 1 puts
-// the hi parts at K 0..7 and the lo parts at K 8..15 of one tile per 32-cell
-// block (one cross-half lane exchange supplies the partner lane's half), the
-// layer-1 C tile is reused as the layer-2 B operand in registers (W2's columns
-// are permuted at packing time to match), and the output layer is 16 local
-// FMAs plus one cross-half shuffle.  Ten matrix-core tiles per wave.  The
-// operand blob (3.5 KB) is read straight from global memory, where it stays
-// cache-resident across the launch.
+// Hidden units are rows (i) and cells are columns (j).  The normalised inputs
+// enter as two-term fp16 splits v = hi + lo (hi = fp16(v), lo = fp16(v - hi),
+// ~22 bits); layer 1 puts the hi parts at K 0..7 and the lo parts at K 8..15 of
+// one tile per 32-cell block (one cross-half lane exchange supplies the partner
+// lane's half).  The layer-1 C tile becomes the layer-2 B operand in registers:
+// relu(c) rounded to fp16 (11 bits, packed convert + packed max, one
+// instruction per value), W2's columns permuted at packing time to match the
+// C layout.  The output layer is 16 local fp32 FMAs plus one cross-half
+// shuffle.  Six matrix-core tiles per wave.  The operand blob (3.5 KB) is read
+// straight from global memory, where it stays cache-resident across the launch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,8 +43,8 @@ constexpr float BETA = 10.0f;
 
 // Matrix-core operand blob, built once per weight update by tlbm_mlp_pack.
 struct Packed {
-  uint16_t a1[64][8];     // layer-1 A: W1 row (lane % 32), inputs 0..6 (both K halves)
-  uint16_t a2[2][64][8];  // layer-2 A, K-steps 0 and 1, columns in C-layout order
+  uint16_t a1[64][8];     // layer-1 A (fp16 bits): W1 row (lane % 32), inputs 0..6 (both K halves)
+  uint16_t a2[2][64][8];  // layer-2 A (fp16 bits), K-steps 0 and 1, columns in C-layout order
   float b1c[2][16];       // layer-1 bias, C layout per lane half
   float b2c[2][16];       // layer-2 bias
   float w3c[2][16];       // output weights
@@ -62,58 +60,57 @@ __host__ __device__ constexpr int c_row(int h, int r) { return 4 * h + (r % 4) +
 // W2 column used by layer-2 K-step s, lane half h, element e
 __host__ __device__ constexpr int w2_col(int s, int h, int e) { return 4 * h + (e % 4) + 16 * s + 8 * (e / 4); }
 
-__device__ __forceinline__ uint16_t bf16_bits(float v) {
-  uint32_t u = __float_as_uint(v);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ bf16x8 as_bf16x8(u16x8 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ f16x8 as_f16x8(u16x8 v) { return __builtin_bit_cast(f16x8, v); }
 
 __device__ __forceinline__ float softplus(float z) {
   const float bz = BETA * z;
   return bz > 20.0f ? z : __logf(1.0f + __expf(bz)) * (1.0f / BETA);
 }
 
-// v = hi + lo with hi = bf16(v), lo = bf16(v - hi) (round to nearest even)
-__device__ __forceinline__ __bf16 split_hi(float v) { return static_cast<__bf16>(v); }
-__device__ __forceinline__ __bf16 split_lo(float v, __bf16 hi) {
-  return static_cast<__bf16>(v - static_cast<float>(hi));
+// max(v, 0) as one integer max on the bit pattern (negative floats are negative
+// integers); exact for every non-NaN v
+__device__ __forceinline__ float relu(float v) {
+  return __builtin_bit_cast(float, __builtin_elementwise_max(__builtin_bit_cast(int, v), 0));
 }
 
-__device__ __forceinline__ bf16x8 xor32(bf16x8 v) {
+__device__ __forceinline__ f16x8 xor32(f16x8 v) {
   u32x4 u = __builtin_bit_cast(u32x4, v);
 #pragma unroll
   for (int i = 0; i < 4; ++i) u[i] = uint32_t(__shfl_xor(int(u[i]), 32, 64));
-  return __builtin_bit_cast(bf16x8, u);
+  return __builtin_bit_cast(f16x8, u);
+}
+
+// relu(c[8s .. 8s+7]) rounded to fp16 (round to nearest even), as a layer-2 B operand
+__device__ __forceinline__ f16x8 act16(const f32x16& c, int s) {
+  f16x8 b;
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    const f32x2 p = {c[8 * s + e], c[8 * s + e + 1]};
+    f16x2 hh = __builtin_convertvector(p, f16x2);
+    hh = __builtin_elementwise_max(hh, (f16x2){_Float16(0), _Float16(0)});
+    b[e] = hh[0];
+    b[e + 1] = hh[1];
+  }
+  return b;
 }
 
 // layer 2 + output for one 32-cell block whose layer-1 tile is c
-__device__ __forceinline__ float block_out(const Packed* P, int h, f32x16 c, bf16x8 a20, bf16x8 a21) {
-  bf16x8 hh0, hl0, hh1, hl1;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const float v0 = fmaxf(c[e], 0.0f), v1 = fmaxf(c[8 + e], 0.0f);
-    hh0[e] = split_hi(v0);
-    hl0[e] = split_lo(v0, hh0[e]);
-    hh1[e] = split_hi(v1);
-    hl1[e] = split_lo(v1, hh1[e]);
-  }
+__device__ __forceinline__ float block_out(const Packed* P, int h, const f32x16& c, f16x8 a20, f16x8 a21) {
   f32x16 d;
 #pragma unroll
   for (int r = 0; r < 16; ++r) d[r] = P->b2c[h][r];
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, hh0, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, hh1, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, hl0, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, hl1, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a20, act16(c, 0), d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a21, act16(c, 1), d, 0, 0, 0);
   float p = 0.0f;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) p += P->w3c[h][r] * fmaxf(d[r], 0.0f);
+  for (int r = 0; r < 16; ++r) p += P->w3c[h][r] * relu(d[r]);
   return p + __shfl_xor(p, 32, 64);
 }
 
@@ -123,31 +120,31 @@ __device__ __forceinline__ float block_out(const Packed* P, int h, f32x16 c, bf1
 __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
   const int lane = __lane_id();
   const int h = lane >> 5;
-  bf16x8 hi, lo;
+  f16x8 hi, lo;
 #pragma unroll
   for (int e = 0; e < 7; ++e) {
     const float v = (x[e] - P->mean[e]) * P->istd[e];
-    hi[e] = split_hi(v);
-    lo[e] = split_lo(v, hi[e]);
+    hi[e] = static_cast<_Float16>(v);
+    lo[e] = static_cast<_Float16>(v - static_cast<float>(hi[e]));
   }
-  hi[7] = lo[7] = static_cast<__bf16>(0.0f);
+  hi[7] = lo[7] = _Float16(0);
   // block 0 (cells of lanes 0..31): K 0..7 = own hi on lanes < 32, K 8..15 = the
   // partner's lo on lanes >= 32; block 1 (cells of lanes 32..63) the other way round
-  const bf16x8 got = xor32(h == 0 ? lo : hi);
-  const bf16x8 b0 = h == 0 ? hi : got;
-  const bf16x8 b1 = h == 0 ? got : lo;
-  const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a1[lane]));
-  const bf16x8 a20 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
-  const bf16x8 a21 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
+  const f16x8 got = xor32(h == 0 ? lo : hi);
+  const f16x8 b0 = h == 0 ? hi : got;
+  const f16x8 b1 = h == 0 ? got : lo;
+  const f16x8 a1 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a1[lane]));
+  const f16x8 a20 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
+  const f16x8 a21 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
   // one 32-cell block at a time keeps a single accumulator tile pair live
   f32x16 c;
 #pragma unroll
   for (int r = 0; r < 16; ++r) c[r] = P->b1c[h][r];
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, c, 0, 0, 0);
   const float p0 = block_out(P, h, c, a20, a21);
 #pragma unroll
   for (int r = 0; r < 16; ++r) c[r] = P->b1c[h][r];
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, c, 0, 0, 0);
   const float p1 = block_out(P, h, c, a20, a21);
   return softplus((h == 0 ? p0 : p1) + P->b3);
 }

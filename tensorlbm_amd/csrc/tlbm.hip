@@ -99,54 +99,21 @@ extern __shared__ __align__(16) unsigned char dyn_lds[];
 #define TLBM_MLP_WAVES 4
 #endif
 
-template <class L, int S, int COLL, bool STREAM, int FEAT>
-__device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
-  using ST = Store<S>;
-  using T = typename ST::T;
-  using C = typename ST::C;
+// Post-stream work of one cell, shared by the one-step and the two-step kernels:
+// Ladd force sample, boundary overrides, full-way bounce-back, collision.
+// `pulled(q, x, y, z)` is the post-stream population q of cell (x, y, z) of the
+// same step (the channel and far-field inlet / outlet read neighbouring cells);
+// PULL_BC = false leaves those two overrides to the caller.
+template <class L, int S, int COLL, bool STREAM, int FEAT, bool PULL_BC, class Pulled>
+__device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], const KArgs<typename Store<S>::C>& a,
+                                            int xx, int yy, int z, uint32_t cell, bool solid, double& fx, double& fy,
+                                            double& fz, const Pulled& pulled) {
+  using C = typename Store<S>::C;
   constexpr int Q = L::Q;
   constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
   constexpr bool kMlp = (FEAT & FEAT_MLP) != 0;
-
-  const uint32_t tile = blockIdx.x;
-  const uint32_t txi = tile % uint32_t(a.tiles_x);
-  const uint32_t rest = tile / uint32_t(a.tiles_x);
-  const uint32_t tyi = rest % uint32_t(a.tiles_y);
-  const int z = int(rest / uint32_t(a.tiles_y));
-  const int x = int(txi * blockDim.x + threadIdx.x);
-  const int y = int(tyi * blockDim.y + threadIdx.y);
-  const bool active = x < a.nx && y < a.ny;
-  if (FEAT == FEAT_NONE && !active) return;
-  // inactive lanes (force path only) work on cell (0, 0, z): in bounds, never stored
-  const int xx = active ? x : 0;
-  const int yy = active ? y : 0;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(yy)) * uint32_t(nx) + uint32_t(xx);
-  const T* fin = static_cast<const T*>(a.fin);
-
-  C f[Q];
-  if constexpr (STREAM) {
-    const int xd = wrap_dn(xx, nx), xu = wrap_up(xx, nx);
-    const int yd = wrap_dn(yy, ny), yu = wrap_up(yy, ny);
-    const int zd = wrap_dn(z, nz), zu = wrap_up(z, nz);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int sx = src_of(L::cx[q], xd, xx, xu);
-      const int sy = src_of(L::cy[q], yd, yy, yu);
-      const int sz = src_of(L::cz[q], zd, z, zu);
-      const T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
-      f[q] = ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
-    }
-  } else {
-    const uint32_t off = uint32_t(yy) * uint32_t(a.isy) + uint32_t(xx);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(ld_off(fin + int64_t(q) * a.isq + int64_t(z) * a.isz, off), q);
-  }
-
-  bool solid = false;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
   if constexpr (kAll) {
-    if (a.solid != nullptr) solid = active && a.solid[cell] != 0;
     if (a.force != nullptr && solid) {
       // Ladd sample: post-stream, pre-bounce-back (obstacles.py:91-130)
 #pragma unroll
@@ -156,17 +123,17 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
         if (L::cz[q] != 0) fz += double(L::cz[q]) * double(f[q]);
       }
     }
-    if (STREAM && a.bc_kind == TLBM_BC_CHANNEL_X) {
+    if (STREAM && PULL_BC && a.bc_kind == TLBM_BC_CHANNEL_X) {
       if (xx == 0) {
         // equilibrium inlet, density of the pulled state at x = 1
         C rho1 = C(0);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) rho1 += pull_one<ST, L>(fin, a, q, 1, yy, z);
+        for (int q = 0; q < Q; ++q) rho1 += pulled(q, 1, yy, z);
         equilibrium<L>(rho1, a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
       } else if (xx == nx - 1) {
         // zero-gradient outlet: copy the pulled state of x = nx-2
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = pull_one<ST, L>(fin, a, q, nx - 2, yy, z);
+        for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
       }
     } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1 && !solid) {
       // Zou-He on the fluid cells of the lid plane; solid cells there keep full-way bounce-back
@@ -180,9 +147,9 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
                            (z == 0 && (fm & TLBM_FACE_ZM)) || (z == nz - 1 && (fm & TLBM_FACE_ZP));
       if (lateral || (xx == 0 && !(fm & TLBM_FACE_XM_PERIODIC))) {
         equilibrium<L>(C(1), a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
-      } else if (xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
+      } else if (PULL_BC && xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = pull_one<ST, L>(fin, a, q, nx - 2, yy, z);
+        for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
       }
     }
     if (solid) {
@@ -265,32 +232,253 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
     }
   }
 
+}
+
+// Q pulled populations of cell (x, y, z) from the input lattice (periodic wrap)
+template <class L, int S>
+__device__ __forceinline__ void gather(typename Store<S>::C (&f)[L::Q], const typename Store<S>::T* fin,
+                                       const KArgs<typename Store<S>::C>& a, int x, int y, int z) {
+  using ST = Store<S>;
+  const int xd = wrap_dn(x, a.nx), xu = wrap_up(x, a.nx);
+  const int yd = wrap_dn(y, a.ny), yu = wrap_up(y, a.ny);
+  const int zd = wrap_dn(z, a.nz), zu = wrap_up(z, a.nz);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int sx = src_of(L::cx[q], xd, x, xu);
+    const int sy = src_of(L::cy[q], yd, y, yu);
+    const int sz = src_of(L::cz[q], zd, z, zu);
+    const typename ST::T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
+    f[q] = ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
+  }
+}
+
+// one FP64 atomic per wave that holds solid cells
+__device__ __forceinline__ void force_flush(double* force, double fx, double fy, double fz, bool any) {
+  if (__any(any)) {
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) {
+      atomicAdd(force + 0, 2.0 * fx);
+      atomicAdd(force + 1, 2.0 * fy);
+      atomicAdd(force + 2, 2.0 * fz);
+    }
+  }
+}
+
+template <class L, int S, int COLL, bool STREAM, int FEAT>
+__device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  using C = typename ST::C;
+  constexpr int Q = L::Q;
+  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
+
+  const uint32_t tile = blockIdx.x;
+  const uint32_t txi = tile % uint32_t(a.tiles_x);
+  const uint32_t rest = tile / uint32_t(a.tiles_x);
+  const uint32_t tyi = rest % uint32_t(a.tiles_y);
+  const int z = int(rest / uint32_t(a.tiles_y));
+  const int x = int(txi * blockDim.x + threadIdx.x);
+  const int y = int(tyi * blockDim.y + threadIdx.y);
+  const bool active = x < a.nx && y < a.ny;
+  if (FEAT == FEAT_NONE && !active) return;
+  // inactive lanes (force / closure paths) work on cell (0, 0, z): in bounds, never stored
+  const int xx = active ? x : 0;
+  const int yy = active ? y : 0;
+  const uint32_t cell = (uint32_t(z) * uint32_t(a.ny) + uint32_t(yy)) * uint32_t(a.nx) + uint32_t(xx);
+  const T* fin = static_cast<const T*>(a.fin);
+
+  C f[Q];
+  if constexpr (STREAM) {
+    gather<L, S>(f, fin, a, xx, yy, z);
+  } else {
+    const uint32_t off = uint32_t(yy) * uint32_t(a.isy) + uint32_t(xx);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(ld_off(fin + int64_t(q) * a.isq + int64_t(z) * a.isz, off), q);
+  }
+  bool solid = false;
+  if constexpr (kAll) {
+    if (a.solid != nullptr) solid = active && a.solid[cell] != 0;
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
+  update_cell<L, S, COLL, STREAM, FEAT, true>(f, a, xx, yy, z, cell, solid, fx, fy, fz, pulled);
   if (active) {
     T* fout = static_cast<T*>(a.fout);
     const uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
 #pragma unroll
     for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
   }
-
   if constexpr (kAll) {
-    if (a.force == nullptr) return;
-    // one FP64 atomic per wave that holds solid cells
-    if (__any(solid)) {
-      fx = wave_sum(fx);
-      fy = wave_sum(fy);
-      fz = wave_sum(fz);
-      if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0) {
-        atomicAdd(a.force + 0, 2.0 * fx);
-        atomicAdd(a.force + 1, 2.0 * fy);
-        atomicAdd(a.force + 2, 2.0 * fz);
-      }
+    if (a.force != nullptr) force_flush(a.force, fx, fy, fz, solid);
+  }
+}
+
+struct Contrib {
+  double fx, fy, fz;
+  bool solid;
+};
+
+// Step one of an inlet / outlet cell of the two-step kernel: the zero-gradient
+// outlet and the inlet density read pulled states of the neighbouring column
+// from HBM.  Out of line so the common path keeps its registers; writes the Q
+// stored values to dst[q * stride] (LDS) and returns the Ladd sample.
+template <class L, int S, int COLL, int FEAT>
+__device__ __attribute__((noinline)) Contrib step_one_edge(const KArgs<typename Store<S>::C>& a, int x, int y, int z,
+                                                           uint32_t cell, typename Store<S>::T* dst, int stride) {
+  using ST = Store<S>;
+  using C = typename ST::C;
+  const typename ST::T* fin = static_cast<const typename ST::T*>(a.fin);
+  C f[L::Q];
+  gather<L, S>(f, fin, a, x, y, z);
+  Contrib r{0.0, 0.0, 0.0, false};
+  if (a.solid != nullptr) r.solid = a.solid[cell] != 0;
+  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
+  update_cell<L, S, COLL, true, FEAT, true>(f, a, x, y, z, cell, r.solid, r.fx, r.fy, r.fz, pulled);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) dst[q * stride] = ST::template put<L>(f[q], q);
+  return r;
+}
+
+// LDS writes complete, then the workgroup barrier.  Unlike __syncthreads() this
+// does not wait for outstanding global loads, so the next plane's prefetch
+// stays in flight across the barrier.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Two fused steps per launch (temporal blocking).  A workgroup owns a TX x TY
+// column of output cells and marches z over `zc` planes.  Step-one states of a
+// (TX+2) x (TY+2) tile (one cell per lane, read from HBM) are kept in LDS for
+// three consecutive planes, in the storage format, so step two sees exactly the
+// values a second launch would read; step two pulls from LDS, so a cell's
+// populations cross HBM once per two steps plus the tile halo.  The step-one
+// loads of the next plane are issued before the LDS phase of the current one.
+template <class L, int S, int COLL, int FEAT, int TX, int TY>
+__device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, int zc) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  using C = typename ST::C;
+  constexpr int Q = L::Q;
+  constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
+  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
+  static_assert((FEAT & FEAT_MLP) == 0, "the closure runs in the one-step kernel");
+  __shared__ T s[3 * Q * EC];  // step-one states of three planes, [slot][q][tile cell]
+  const int t = threadIdx.x;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int x0 = int(blockIdx.x) * TX, y0 = int(blockIdx.y) * TY, z0 = int(blockIdx.z) * zc;
+  const int zend = z0 + zc < nz ? z0 + zc : nz;
+  const T* fin = static_cast<const T*>(a.fin);
+  // step-one cell of this lane (tile-local lx, ly; global gx, gy)
+  const bool h1 = t < EC;
+  const int lx = h1 ? t % EX : 0, ly = h1 ? t / EX : 0;
+  const int gx0 = x0 - 1 + lx, gy0 = y0 - 1 + ly;
+  const int gx = (gx0 % nx + nx) % nx;  // halo lanes of a tile wider than the lattice wrap more than once
+  const int gy = (gy0 % ny + ny) % ny;
+  const bool own1 = h1 && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && gx0 < nx && gy0 < ny;
+  const bool edge_bc = kAll && (gx == 0 || gx == nx - 1) &&
+                       (a.bc_kind == TLBM_BC_CHANNEL_X || a.bc_kind == TLBM_BC_FAR_FIELD);
+  // output cell of this lane
+  const bool h2 = t < TX * TY;
+  const int ox = h2 ? t % TX : 0, oy = h2 ? t / TX : 0;
+  const int ux_ = x0 + ox, uy_ = y0 + oy;
+  const bool st = h2 && ux_ < nx && uy_ < ny;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  bool any_solid = false;
+
+  C pre[Q];  // pulled populations of this lane's step-one cell on the next plane
+  bool pre_solid = false;
+  auto prefetch = [&](int p) {
+    if (!h1) return;
+    const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
+    gather<L, S>(pre, fin, a, gx, gy, pz);
+    if constexpr (kAll) {
+      if (a.solid != nullptr)
+        pre_solid = a.solid[(uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx)] != 0;
     }
+  };
+  auto step_one = [&](int p, int slot, bool owned_plane) {
+    if (!h1) return;
+    const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
+    const uint32_t cell = (uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx);
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    bool solid = false;
+    if (kAll && edge_bc) {
+      const Contrib r = step_one_edge<L, S, COLL, FEAT>(a, gx, gy, pz, cell, s + size_t(slot) * Q * EC + t, EC);
+      dx = r.fx;
+      dy = r.fy;
+      dz = r.fz;
+      solid = r.solid;
+    } else {
+      C f[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = pre[q];
+      solid = pre_solid;
+      auto pulled = [&](int q, int px, int py, int qz) { return pull_one<ST, L>(fin, a, q, px, py, qz); };
+      update_cell<L, S, COLL, true, FEAT, false>(f, a, gx, gy, pz, cell, solid, dx, dy, dz, pulled);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) s[(slot * Q + q) * EC + t] = ST::template put<L>(f[q], q);
+    }
+    if (own1 && owned_plane) {
+      fx += dx;
+      fy += dy;
+      fz += dz;
+      any_solid |= solid;
+    }
+  };
+
+  // iteration k computes step one on plane z0 + k + 1 (slot (k + 2) % 3) and, from
+  // k = 0 on, step two on plane z = z0 + k; k = -2 and -1 fill the first two planes
+  prefetch(z0 - 1);
+  for (int k = -2; z0 + k < zend; ++k) {
+    const int z = z0 + k;
+    const int p1 = z + 1;
+    step_one(p1, (k + 2) % 3, p1 >= z0 && p1 < zend);
+    if (p1 < zend) prefetch(p1 + 1);  // the next iteration's step-one plane
+    lds_barrier();
+    if (k >= 0 && st) {
+      // step-one value of population q pulled into cell (x, y) of plane z (same
+      // plane only: the boundary overrides read along x)
+      auto lds_at = [&](int q, int x, int y, int) -> C {
+        const int slot = (k + 1 - L::cz[q] + 3) % 3;
+        const int sx = x - L::cx[q] - (x0 - 1), sy = y - L::cy[q] - (y0 - 1);
+        return ST::template get<L>(s[(slot * Q + q) * EC + sy * EX + sx], q);
+      };
+      C f[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = lds_at(q, ux_, uy_, z);
+      const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(uy_)) * uint32_t(nx) + uint32_t(ux_);
+      bool solid = false;
+      if constexpr (kAll) {
+        if (a.solid != nullptr) solid = a.solid[cell] != 0;
+      }
+      update_cell<L, S, COLL, true, FEAT, true>(f, a, ux_, uy_, z, cell, solid, fx, fy, fz, lds_at);
+      any_solid |= solid;
+      T* fout = static_cast<T*>(a.fout);
+      const uint32_t off = uint32_t(uy_) * uint32_t(a.osy) + uint32_t(ux_);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
+    }
+    lds_barrier();
+  }
+  if constexpr (kAll) {
+    if (a.force != nullptr) force_flush(a.force, fx, fy, fz, any_solid);
   }
 }
 
 template <class L, int S, int COLL, bool STREAM, int FEAT>
 __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
   lbm_body<L, S, COLL, STREAM, FEAT>(a);
+}
+
+// two-step tile: TX x TY output cells, (TX+2)(TY+2) lanes rounded to whole waves
+template <int TX, int TY>
+constexpr int tb_threads() {
+  return ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+}
+
+template <class L, int S, int COLL, int FEAT, int TX, int TY>
+__global__ __launch_bounds__((tb_threads<TX, TY>())) void k_lbm2(const KArgs<typename Store<S>::C> a, int zc) {
+  lbm2_body<L, S, COLL, FEAT, TX, TY>(a, zc);
 }
 
 // The learned-closure variants hold the matrix-core tiles next to the Q
@@ -644,6 +832,54 @@ int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb
   return check_launch("k_lbm");
 }
 
+// ---- two steps per launch ----------------------------------------------------
+
+// tile of the two-step kernel: 64 x 6 output cells; 32 x 8 where three planes
+// of 64 x 6 would not fit the 160 KB of LDS (D3Q27, 4-byte storage)
+template <class L, int S>
+struct Tile2 {
+  static constexpr bool wide = 3 * L::Q * 66 * 8 * sizeof(typename Store<S>::T) <= 160 * 1024;
+  static constexpr int TX = wide ? 64 : 32;
+  static constexpr int TY = wide ? 6 : 8;
+};
+
+template <class L, int S>
+constexpr bool step2_store() {
+  return L::Q != 9 && (S == ST_F32 || S == ST_F16_DEV || S == ST_BF16_DEV);
+}
+
+// z planes per workgroup: 32, fewer while the grid is below four workgroups per CU
+inline int step2_zc(int cols, int nz) {
+  int zc = 32;
+  while (zc > 8 && int64_t(cols) * ((nz + zc - 1) / zc) < 1024) zc /= 2;
+  return zc;
+}
+
+// why a two-step launch cannot serve these arguments (nullptr: it can)
+inline const char* step2_reason(int lattice, int store, const tlbm_collision* coll, const tlbm_boundary* bc, int nx) {
+  if (lattice == TLBM_D2Q9) return "two-step launches are built for D3Q19 and D3Q27";
+  if (store != TLBM_STORE_F32 && store != TLBM_STORE_F16_DEV && store != TLBM_STORE_BF16_DEV)
+    return "two-step launches take float32 or 16-bit deviation storage";
+  if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return "collision parameters required";
+  if (coll->tau_mode == TLBM_TAU_MLP) return "the learned closure runs in the one-step kernel";
+  if (bc != nullptr && (bc->kind == TLBM_BC_CHANNEL_X || bc->kind == TLBM_BC_FAR_FIELD)) {
+    // the outlet copies x = nx-2, whose pulls reach x = nx-3: same tile as x = nx-1
+    const int tx = lattice == TLBM_D3Q27 && store == TLBM_STORE_F32 ? 32 : 64;
+    if (nx < 4 || (nx - 1) % tx < 2) return "the outlet column would straddle two tiles";
+  }
+  return nullptr;
+}
+
+template <class L, int S, int COLL, int FEAT>
+int launch_lbm2(const KArgs<typename Store<S>::C>& a, hipStream_t s) {
+  constexpr int TX = Tile2<L, S>::TX, TY = Tile2<L, S>::TY;
+  const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY;
+  const int zc = step2_zc(gx * gy, a.nz);
+  const dim3 grid(gx, gy, (a.nz + zc - 1) / zc);
+  hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, TX, TY>), grid, dim3(tb_threads<TX, TY>()), 0, s, a, zc);
+  return check_launch("k_lbm2");
+}
+
 template <class L, int COLL>
 constexpr bool coll_built() {
   if (COLL == TLBM_COLL_NONE || COLL == TLBM_COLL_BGK || COLL == TLBM_COLL_MRT) return true;
@@ -656,6 +892,7 @@ constexpr bool coll_built() {
 struct Call {
   int lattice, store, kind;
   bool stream_, feat;
+  int steps;
   const void* fin;
   const int64_t* is;
   void* fout;
@@ -686,6 +923,14 @@ int run_coll(const Call& c) {
         return launch_lbm<L, S, COLL, true, FEAT_MLP>(a, bx, by, nb, c.s);
       } else {
         return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
+      }
+    }
+    if (c.steps == 2) {
+      if constexpr (step2_store<L, S>() && COLL != TLBM_COLL_NONE) {
+        if (c.feat) return launch_lbm2<L, S, COLL, FEAT_ALL>(a, c.s);
+        return launch_lbm2<L, S, COLL, FEAT_NONE>(a, c.s);
+      } else {
+        return fail(TLBM_ERR_UNSUPPORTED, "two-step launch not built for this lattice / storage");
       }
     }
     if (c.stream_) {
@@ -840,9 +1085,30 @@ int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_stride
   if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
   const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
   if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
-  Call c{lattice, store, coll->kind, true, feat, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
+  Call c{lattice, store, coll->kind, true, feat, 1, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
          (hipStream_t)stream};
   return run(c);
+}
+
+int tlbm_step2(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
+               const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll, const int8_t* solid,
+               const tlbm_boundary* bc, double* force, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  int64_t is[3], os[3];
+  int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
+  if (rc) return rc;
+  if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
+  if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
+  if (const char* why = step2_reason(lattice, store, coll, bc, nx)) return fail(TLBM_ERR_UNSUPPORTED, "%s", why);
+  const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
+  if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
+  Call c{lattice, store, coll->kind, true, feat, 2, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
+         (hipStream_t)stream};
+  return run(c);
+}
+
+int tlbm_step2_supported(int lattice, int store, const tlbm_collision* coll, const tlbm_boundary* bc, int nx) {
+  return step2_reason(lattice, store, coll, bc, nx) == nullptr ? 1 : 0;
 }
 
 int tlbm_stream(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
@@ -852,7 +1118,7 @@ int tlbm_stream(int lattice, int store, const void* f_in, const int64_t* in_stri
   int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
   if (rc) return rc;
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; streaming needs two buffers");
-  Call c{lattice, store, TLBM_COLL_NONE, true, false, f_in, is, f_out, os, nz, ny, nx, nullptr, nullptr, nullptr,
+  Call c{lattice, store, TLBM_COLL_NONE, true, false, 1, f_in, is, f_out, os, nz, ny, nx, nullptr, nullptr, nullptr,
          nullptr, (hipStream_t)stream};
   return run(c);
 }
@@ -867,7 +1133,7 @@ int tlbm_collide(int lattice, int store, const void* f_in, const int64_t* in_str
   int rc = prepare(lattice, store, f_in, in_strides, f_out, out_strides, nz, ny, nx, is, os);
   if (rc) return rc;
   if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return fail(TLBM_ERR_INVALID, "collision kind required");
-  Call c{lattice, store, coll->kind, false, false, f_in, is, f_out, os, nz, ny, nx, coll, nullptr, nullptr, nullptr,
+  Call c{lattice, store, coll->kind, false, false, 1, f_in, is, f_out, os, nz, ny, nx, coll, nullptr, nullptr, nullptr,
          (hipStream_t)stream};
   return run(c);
 }

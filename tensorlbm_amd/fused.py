@@ -88,6 +88,18 @@ def _launch_step(f, out, coll, solid=None, bc=None, force=None, *, deviation=Fal
     return out
 
 
+def _launch_step2(f, out, coll, solid=None, bc=None, force=None, *, deviation=False, what="step2"):
+    nat.require_gpu(f, what)
+    if f.stride(-1) != 1:
+        f = f.contiguous()
+    q, nz, ny, nx = _dims(f)
+    store = nat.store_code(f.dtype, deviation)
+    rc = nat.lib().tlbm_step2(q, store, f.data_ptr(), nat.strides_of(f), out.data_ptr(), nat.strides_of(out), nz, ny,
+                              nx, coll, nat.ptr(solid), bc, nat.ptr(force), nat.stream_of(f))
+    nat.check(rc, what)
+    return out
+
+
 def fused_step(f: torch.Tensor, tau: float, *, out: torch.Tensor | None = None, collision: str = "BGK",
                rates: Sequence[float] | None = None, deviation: bool = False) -> torch.Tensor:
     """Periodic pull-stream + collide in one launch (triton_fused.py:396-436 order: stream, then collide).
@@ -224,6 +236,11 @@ class FusedSolver3D:
     ``closure=EddyViscosityMLP3D`` relaxes D3Q19 BGK with the learned eddy
     viscosity, evaluated inside the step (``C_s`` then sets the strain's time
     scale: the Smagorinsky τ of that constant, or τ itself for ``C_s=0``).
+
+    ``run()`` advances pairs of steps with one launch each where the library
+    builds that case (``temporal_blocking=True``, the default): the same
+    results as single steps, one HBM round trip of the populations per two
+    steps.  ``step2()`` is one such pair.
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
@@ -231,7 +248,8 @@ class FusedSolver3D:
                  collision: str = "BGK", rates: Sequence[float] | None = None, deviation: bool = False,
                  C_s: float = 0.0, delta: float = 1.0, solid: torch.Tensor | None = None,
                  boundary: str | None = None, u_bc=(0.0, 0.0, 0.0), layout: str = "padded",
-                 closure=None, bc_config: dict | None = None, **_tiles) -> None:
+                 closure=None, bc_config: dict | None = None, temporal_blocking: bool = True,
+                 **_tiles) -> None:
         if precision is not None:
             policy = PrecisionPolicy.parse(precision)
             if policy.compute_dtype != torch.float32:
@@ -294,28 +312,50 @@ class FusedSolver3D:
         self._buf_a: torch.Tensor | None = None
         self._buf_b: torch.Tensor | None = None
         self._report_step = 0
+        self._two_step = bool(temporal_blocking) and lat != "D2Q9" and bool(
+            nat.lib().tlbm_step2_supported(self._Q, self._store, self._coll, self._bc, self.nx))
 
     def _shape(self):
         return (self._Q, self.ny, self.nx) if self.lattice == "D2Q9" else (self._Q, self.nz, self.ny, self.nx)
 
-    def step(self, f: torch.Tensor, *, force: torch.Tensor | None = None) -> torch.Tensor:
-        """Advance one step; returns a persistent internal buffer (never ``f`` itself)."""
+    @property
+    def two_step(self) -> bool:
+        """True when ``run()`` / ``step2()`` advance two steps per launch."""
+        return self._two_step
+
+    def _check_input(self, f: torch.Tensor) -> None:
         if f.dtype != self.dtype:
             pol = f" (precision {self.precision_policy.name})" if self.precision_policy is not None else ""
             raise ValueError(f"step() input dtype {f.dtype} does not match the solver's storage dtype "
                              f"{self.dtype}{pol}; store the initial distribution in the storage dtype first")
         if tuple(f.shape) != self._shape():
             raise ValueError(f"step() input shape {tuple(f.shape)} != {self._shape()}")
+
+    def _target(self, f: torch.Tensor, force: torch.Tensor | None) -> torch.Tensor:
         if self._buf_a is None:
             self._buf_a = self._alloc()
             self._buf_b = self._alloc()
-        out = self._buf_b if f.data_ptr() == self._buf_a.data_ptr() else self._buf_a
         if force is not None and self.solid is None:
             raise ValueError("force needs a solid mask")
+        return self._buf_b if f.data_ptr() == self._buf_a.data_ptr() else self._buf_a
+
+    def step(self, f: torch.Tensor, *, force: torch.Tensor | None = None) -> torch.Tensor:
+        """Advance one step; returns a persistent internal buffer (never ``f`` itself)."""
+        self._check_input(f)
+        out = self._target(f, force)
         if self.closure is not None:
             self._coll.mlp = self.closure.packed.data_ptr()
         return _launch_step(f, out, self._coll, solid=self.solid, bc=self._bc, force=force,
                             deviation=self.deviation, what="FusedSolver3D.step")
+
+    def step2(self, f: torch.Tensor, *, force: torch.Tensor | None = None) -> torch.Tensor:
+        """Advance two steps (one launch when ``two_step``); ``force`` receives both steps' samples."""
+        if not self._two_step:
+            return self.step(self.step(f, force=force), force=force)
+        self._check_input(f)
+        out = self._target(f, force)
+        return _launch_step2(f, out, self._coll, solid=self.solid, bc=self._bc, force=force,
+                             deviation=self.deviation, what="FusedSolver3D.step2")
 
     def _alloc(self) -> torch.Tensor:
         if self.layout == "contiguous":
@@ -325,9 +365,16 @@ class FusedSolver3D:
     def run(self, f: torch.Tensor, n_steps: int, *, reporters: Sequence | None = None):
         """Advance ``n_steps``; reporters receive a StepContext after every step (lbm_step.py:615-686)."""
         if not reporters:
-            for _ in range(int(n_steps)):
-                f = self.step(f)
-            self._report_step += int(n_steps)
+            n = int(n_steps)
+            if self._two_step:
+                for _ in range(n // 2):
+                    f = self.step2(f)
+                if n % 2:
+                    f = self.step(f)
+            else:
+                for _ in range(n):
+                    f = self.step(f)
+            self._report_step += n
             return f, [{} for _ in range(int(n_steps))]
         from .reporters import StepContext, dispatch
 
