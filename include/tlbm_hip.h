@@ -135,10 +135,11 @@ int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_stride
  * Two consecutive tlbm_step()s in one launch, f_in -> f_out (temporal blocking: the
  * intermediate lattice stays in LDS).  Replaces two iterations of the reporter-free
  * run() loop, src/tensorlbm/triton_fused.py:892-895.  Same arguments and results as
- * tlbm_step(f_in -> tmp) then tlbm_step(tmp -> f_out); force receives the sum of
- * both steps' samples.  D3Q19 / D3Q27; F32, F16_DEV, BF16_DEV storage; every
- * collision and boundary except TAU_MLP.  Returns TLBM_ERR_UNSUPPORTED for other
- * arguments, which tlbm_step2_supported() reports without launching.
+ * tlbm_step(f_in -> tmp) then tlbm_step(tmp -> f_out), up to rounding.  D3Q19 /
+ * D3Q27; F32, F16_DEV, BF16_DEV storage; every collision and boundary except
+ * TAU_MLP.  force must be NULL: sample the Ladd force with tlbm_step.  Returns
+ * TLBM_ERR_UNSUPPORTED for other arguments, which tlbm_step2_supported()
+ * reports (force aside) without launching.
  */
 int tlbm_step2(int lattice, int store, const void* f_in, const int64_t* in_strides, void* f_out,
                const int64_t* out_strides, int nz, int ny, int nx, const tlbm_collision* coll,

@@ -144,10 +144,13 @@ __device__ __forceinline__ C pi_norm(const C (&pi)[6]) {
               C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
 }
 
-// Hou et al. tau_eff clamped to [0.5001, max(1, tau+0.5)]  (turbulence.py:183-208)
+// Hou et al. tau_eff clamped to [0.5001, max(1, tau+0.5)]  (turbulence.py:183-208).
+// tau^2 enters through an explicit fma: as a product of launch constants it would
+// otherwise be fused or not depending on whether the compiler hoists it out of a
+// loop, and kernels sharing this code must round identically.
 template <typename C>
 __device__ __forceinline__ C smagorinsky_tau(C pn, C rho_safe, C tau, C cs_delta_sq, C tau_cap) {
-  C disc = tau * tau + C(18) * cs_delta_sq * pn / rho_safe;
+  C disc = fma(tau, tau, C(18) * cs_delta_sq * pn / rho_safe);
   disc = disc > C(0) ? disc : C(0);
   C te = C(0.5) * (tau + sqrt(disc));
   te = te < C(0.5001) ? C(0.5001) : te;
@@ -162,7 +165,7 @@ __device__ __forceinline__ C smagorinsky_omega(C pn, C rho_safe, C tau, C cs_del
 // Unclamped closure of collide_cumulant_d3q19 (cumulant.py:1316-1321)
 template <typename C>
 __device__ __forceinline__ C smagorinsky_raw_omega(C pn, C rho_safe, C tau, C cs_delta_sq) {
-  return C(2) / (tau + sqrt(tau * tau + C(18) * cs_delta_sq * pn / rho_safe));
+  return C(2) / (tau + sqrt(fma(tau, tau, C(18) * cs_delta_sq * pn / rho_safe)));
 }
 
 // 4.5 w_q (H_q : Pi) with doubled off-diagonals (cumulant.py:1331-1362)

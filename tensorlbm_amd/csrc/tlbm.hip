@@ -36,9 +36,20 @@
 
 namespace tlbm {
 
-// FEAT_ALL: solid mask / force / boundaries; FEAT_MLP: learned closure (whole
-// waves stay alive for the matrix-core evaluation).  Bits combine.
-enum Feat { FEAT_NONE = 0, FEAT_ALL = 1, FEAT_MLP = 2, FEAT_MLP_ALL = 3 };
+// FEAT_BC: solid mask and boundaries; FEAT_FORCE: Ladd force sample; FEAT_MLP:
+// learned closure (whole waves stay alive for the matrix-core evaluation).
+// Bits combine; the one-step kernel builds FEAT_ALL, the two-step kernel FEAT_BC.
+enum Feat {
+  FEAT_NONE = 0,
+  FEAT_BC = 1,
+  FEAT_MLP = 2,
+  FEAT_FORCE = 4,
+  FEAT_ALL = FEAT_BC | FEAT_FORCE,
+  FEAT_MLP_ALL = FEAT_MLP | FEAT_ALL
+};
+
+// boundary kind known at compile time (a TLBM_BC_* value) or read from the launch
+constexpr int BCK_RUNTIME = -1;
 
 template <typename C>
 struct KArgs {
@@ -64,9 +75,22 @@ __device__ __forceinline__ T ld_off(const T* base, uint32_t off) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off * uint32_t(sizeof(T)));
 }
 
+// A pointer the caller knows to be uniform over the wave, moved to scalar registers
+// so that the memory op uses the scalar-base + 32-bit-offset form (otherwise the
+// compiler may build a 64-bit per-lane address with quarter-rate multiplies).
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+  return reinterpret_cast<T*>((uint64_t(hi) << 32) | lo);
+}
+
+// non-temporal store to a wave-uniform base plus a per-lane 32-bit element offset
 template <typename T>
 __device__ __forceinline__ void st_nt(T* base, uint32_t off, T v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off * uint32_t(sizeof(T))));
+  __builtin_nontemporal_store(
+      v, reinterpret_cast<T*>(reinterpret_cast<char*>(uniform_ptr(base)) + off * uint32_t(sizeof(T))));
 }
 
 __device__ __forceinline__ int wrap_dn(int v, int n) { return v == 0 ? n - 1 : v - 1; }
@@ -99,46 +123,97 @@ extern __shared__ __align__(16) unsigned char dyn_lds[];
 #define TLBM_MLP_WAVES 4
 #endif
 
+// Equilibrium of the boundary velocity u (a launch constant) computed where it
+// is used.  The empty asm makes the compiler treat the velocity as produced at
+// this point: otherwise it hoists the Q equilibrium values out of the z loop of
+// the two-step kernel and keeps them in registers for its whole run.
+template <class L, typename C>
+__device__ __forceinline__ void equilibrium_here(C rho, const C (&u)[3], C (&f)[L::Q]) {
+  C ux = u[0], uy = u[1], uz = u[2];
+  asm volatile("" : "+v"(ux), "+v"(uy), "+v"(uz));
+  equilibrium<L>(rho, ux, uy, uz, f);
+}
+
+// How update_cell sees the pulled states of the neighbouring column that the
+// channel inlet (x = 1) and the zero-gradient outlets (x = nx-2) read:
+//   BCN_PULL : through the callback pulled(q, x, y, z) (one-step kernel, step
+//              two of the two-step kernel)
+//   BCN_IN_F : on those edge cells (`nb_edge`) f already holds the neighbouring
+//              column's pulled state, gathered there instead of at the cell
+//              (step one of the two-step kernel: no extra loads, no extra registers)
+enum BcNeighbour { BCN_PULL = 0, BCN_IN_F = 1 };
+
+// Ladd sample of cell (x, y, z)'s own pulled populations read from the input
+// lattice, for the rare solid edge cell whose f holds its neighbour's state
+// (rolled loop: kept off the register budget of the common path)
+template <class L, int S>
+__device__ __forceinline__ void own_force_sample(const KArgs<typename Store<S>::C>& a, int x, int y, int z,
+                                                 double& fx, double& fy, double& fz) {
+  const typename Store<S>::T* fin = static_cast<const typename Store<S>::T*>(a.fin);
+#pragma unroll 1
+  for (int q = 1; q < L::Q; ++q) {
+    const double v = double(pull_one<Store<S>, L>(fin, a, q, x, y, z));
+    if (L::cx[q] != 0) fx += double(L::cx[q]) * v;
+    if (L::cy[q] != 0) fy += double(L::cy[q]) * v;
+    if (L::cz[q] != 0) fz += double(L::cz[q]) * v;
+  }
+}
+
 // Post-stream work of one cell, shared by the one-step and the two-step kernels:
 // Ladd force sample, boundary overrides, full-way bounce-back, collision.
-// `pulled(q, x, y, z)` is the post-stream population q of cell (x, y, z) of the
-// same step (the channel and far-field inlet / outlet read neighbouring cells);
-// PULL_BC = false leaves those two overrides to the caller.
-template <class L, int S, int COLL, bool STREAM, int FEAT, bool PULL_BC, class Pulled>
+//
+// `eq_unit(e)` fills e with f_eq(rho = 1, bc_u), from which the inlet takes
+// rho_1 * e (f_eq is linear in rho) and the far-field faces take e itself.
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCN, int BCK, class Pulled, class EqUnit>
 __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], const KArgs<typename Store<S>::C>& a,
-                                            int xx, int yy, int z, uint32_t cell, bool solid, double& fx, double& fy,
-                                            double& fz, const Pulled& pulled) {
+                                            int xx, int yy, int z, uint32_t cell, bool solid, bool nb_edge,
+                                            double& fx, double& fy, double& fz, const Pulled& pulled,
+                                            const EqUnit& eq_unit) {
   using C = typename Store<S>::C;
   constexpr int Q = L::Q;
-  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
+  constexpr bool kAll = (FEAT & FEAT_BC) != 0;
+  constexpr bool kForce = (FEAT & FEAT_FORCE) != 0;
   constexpr bool kMlp = (FEAT & FEAT_MLP) != 0;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
+  // a compile-time kind folds the other boundaries away
+  const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
   if constexpr (kAll) {
-    if (a.force != nullptr && solid) {
+    if (kForce && a.force != nullptr && solid) {
       // Ladd sample: post-stream, pre-bounce-back (obstacles.py:91-130)
+      if (BCN == BCN_IN_F && nb_edge) {
+        own_force_sample<L, S>(a, xx, yy, z, fx, fy, fz);
+      } else {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        if (L::cx[q] != 0) fx += double(L::cx[q]) * double(f[q]);
-        if (L::cy[q] != 0) fy += double(L::cy[q]) * double(f[q]);
-        if (L::cz[q] != 0) fz += double(L::cz[q]) * double(f[q]);
+        for (int q = 0; q < Q; ++q) {
+          if (L::cx[q] != 0) fx += double(L::cx[q]) * double(f[q]);
+          if (L::cy[q] != 0) fy += double(L::cy[q]) * double(f[q]);
+          if (L::cz[q] != 0) fz += double(L::cz[q]) * double(f[q]);
+        }
       }
     }
-    if (STREAM && PULL_BC && a.bc_kind == TLBM_BC_CHANNEL_X) {
+    if (STREAM && bck == TLBM_BC_CHANNEL_X) {
       if (xx == 0) {
         // equilibrium inlet, density of the pulled state at x = 1
         C rho1 = C(0);
+        if constexpr (BCN == BCN_PULL) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) rho1 += pulled(q, 1, yy, z);
-        equilibrium<L>(rho1, a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
-      } else if (xx == nx - 1) {
+          for (int q = 0; q < Q; ++q) rho1 += pulled(q, 1, yy, z);
+        } else {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) rho1 += f[q];
+        }
+        eq_unit(f);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] *= rho1;
+      } else if (BCN == BCN_PULL && xx == nx - 1) {
         // zero-gradient outlet: copy the pulled state of x = nx-2
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
       }
-    } else if (STREAM && a.bc_kind == TLBM_BC_LID_Y && yy == ny - 1 && !solid) {
+    } else if (STREAM && bck == TLBM_BC_LID_Y && yy == ny - 1 && !solid) {
       // Zou-He on the fluid cells of the lid plane; solid cells there keep full-way bounce-back
       if constexpr (Q == 19) zou_he_lid(f, a.bc_u[0]);
-    } else if (STREAM && a.bc_kind == TLBM_BC_FAR_FIELD) {
+    } else if (STREAM && bck == TLBM_BC_FAR_FIELD) {
       // far_field_bc_3d (boundaries3d.py:446-537): free-stream equilibrium on the
       // inlet and the selected lateral faces (lateral faces win on edges),
       // zero-gradient outlet
@@ -146,19 +221,23 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       const bool lateral = (yy == 0 && (fm & TLBM_FACE_YM)) || (yy == ny - 1 && (fm & TLBM_FACE_YP)) ||
                            (z == 0 && (fm & TLBM_FACE_ZM)) || (z == nz - 1 && (fm & TLBM_FACE_ZP));
       if (lateral || (xx == 0 && !(fm & TLBM_FACE_XM_PERIODIC))) {
-        equilibrium<L>(C(1), a.bc_u[0], a.bc_u[1], a.bc_u[2], f);
-      } else if (PULL_BC && xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
+        eq_unit(f);
+      } else if (BCN == BCN_PULL && xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
       }
     }
-    if (solid) {
+    // full-way bounce-back; the wave vote skips the (if-converted) swaps in the
+    // waves that hold no solid cell, i.e. almost all of them
+    if (__any(solid)) {
+      if (solid) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        if (q < L::opp[q]) {
-          const C t = f[q];
-          f[q] = f[L::opp[q]];
-          f[L::opp[q]] = t;
+        for (int q = 0; q < Q; ++q) {
+          if (q < L::opp[q]) {
+            const C t = f[q];
+            f[q] = f[L::opp[q]];
+            f[L::opp[q]] = t;
+          }
         }
       }
     }
@@ -272,7 +351,7 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   using T = typename ST::T;
   using C = typename ST::C;
   constexpr int Q = L::Q;
-  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
+  constexpr bool kAll = (FEAT & FEAT_BC) != 0;
 
   const uint32_t tile = blockIdx.x;
   const uint32_t txi = tile % uint32_t(a.tiles_x);
@@ -303,42 +382,18 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   }
   double fx = 0.0, fy = 0.0, fz = 0.0;
   auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
-  update_cell<L, S, COLL, STREAM, FEAT, true>(f, a, xx, yy, z, cell, solid, fx, fy, fz, pulled);
+  auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
+  update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK_RUNTIME>(f, a, xx, yy, z, cell, solid, false, fx, fy, fz,
+                                                                pulled, eq_unit);
   if (active) {
     T* fout = static_cast<T*>(a.fout);
     const uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
 #pragma unroll
     for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
   }
-  if constexpr (kAll) {
+  if constexpr ((FEAT & FEAT_FORCE) != 0) {
     if (a.force != nullptr) force_flush(a.force, fx, fy, fz, solid);
   }
-}
-
-struct Contrib {
-  double fx, fy, fz;
-  bool solid;
-};
-
-// Step one of an inlet / outlet cell of the two-step kernel: the zero-gradient
-// outlet and the inlet density read pulled states of the neighbouring column
-// from HBM.  Out of line so the common path keeps its registers; writes the Q
-// stored values to dst[q * stride] (LDS) and returns the Ladd sample.
-template <class L, int S, int COLL, int FEAT>
-__device__ __attribute__((noinline)) Contrib step_one_edge(const KArgs<typename Store<S>::C>& a, int x, int y, int z,
-                                                           uint32_t cell, typename Store<S>::T* dst, int stride) {
-  using ST = Store<S>;
-  using C = typename ST::C;
-  const typename ST::T* fin = static_cast<const typename ST::T*>(a.fin);
-  C f[L::Q];
-  gather<L, S>(f, fin, a, x, y, z);
-  Contrib r{0.0, 0.0, 0.0, false};
-  if (a.solid != nullptr) r.solid = a.solid[cell] != 0;
-  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
-  update_cell<L, S, COLL, true, FEAT, true>(f, a, x, y, z, cell, r.solid, r.fx, r.fy, r.fz, pulled);
-#pragma unroll
-  for (int q = 0; q < L::Q; ++q) dst[q * stride] = ST::template put<L>(f[q], q);
-  return r;
 }
 
 // LDS writes complete, then the workgroup barrier.  Unlike __syncthreads() this
@@ -353,16 +408,21 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // values a second launch would read; step two pulls from LDS, so a cell's
 // populations cross HBM once per two steps plus the tile halo.  The step-one
 // loads of the next plane are issued before the LDS phase of the current one.
-template <class L, int S, int COLL, int FEAT, int TX, int TY>
+template <class L, int S, int COLL, int FEAT, int BCK, int TX, int TY>
 __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, int zc) {
   using ST = Store<S>;
   using T = typename ST::T;
   using C = typename ST::C;
   constexpr int Q = L::Q;
   constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
-  constexpr bool kAll = (FEAT & FEAT_ALL) != 0;
-  static_assert((FEAT & FEAT_MLP) == 0, "the closure runs in the one-step kernel");
+  constexpr bool kAll = (FEAT & FEAT_BC) != 0;
+  static_assert((FEAT & (FEAT_MLP | FEAT_FORCE)) == 0, "the closure and the force sample run in the one-step kernel");
   __shared__ T s[3 * Q * EC];  // step-one states of three planes, [slot][q][tile cell]
+  // solid flags of the same tile cells: step two takes them from here rather than
+  // waiting on a mask load of its own each plane
+  __shared__ uint8_t s_solid[kAll ? 3 * EC : 1];
+  // f_eq(1, bc_u), built once per workgroup with the code the one-step kernel runs
+  __shared__ C s_eq[kAll ? Q : 1];
   const int t = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int x0 = int(blockIdx.x) * TX, y0 = int(blockIdx.y) * TY, z0 = int(blockIdx.z) * zc;
@@ -374,56 +434,62 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   const int gx0 = x0 - 1 + lx, gy0 = y0 - 1 + ly;
   const int gx = (gx0 % nx + nx) % nx;  // halo lanes of a tile wider than the lattice wrap more than once
   const int gy = (gy0 % ny + ny) % ny;
-  const bool own1 = h1 && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && gx0 < nx && gy0 < ny;
-  const bool edge_bc = kAll && (gx == 0 || gx == nx - 1) &&
-                       (a.bc_kind == TLBM_BC_CHANNEL_X || a.bc_kind == TLBM_BC_FAR_FIELD);
+  // inlet / outlet cells read their neighbouring column's pulled state in step
+  // one: gather there instead (see BCN_IN_F)
+  bool nb_edge = false;
+  if constexpr (kAll) {
+    const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
+    nb_edge = (bck == TLBM_BC_CHANNEL_X && (gx == 0 || gx == nx - 1)) ||
+              (bck == TLBM_BC_FAR_FIELD && gx == nx - 1 && !(a.bc_faces & TLBM_FACE_XP_PERIODIC));
+  }
+  const int gxn = nb_edge ? (gx == 0 ? 1 : nx - 2) : gx;
   // output cell of this lane
   const bool h2 = t < TX * TY;
   const int ox = h2 ? t % TX : 0, oy = h2 ? t / TX : 0;
   const int ux_ = x0 + ox, uy_ = y0 + oy;
   const bool st = h2 && ux_ < nx && uy_ < ny;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  bool any_solid = false;
+  double unused_force = 0.0;  // no force sample here: tlbm_step2 takes no force buffer
+
+  if constexpr (kAll) {
+    if (t == 0) {
+      C e[Q];
+      equilibrium_here<L>(C(1), a.bc_u, e);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) s_eq[q] = e[q];
+    }
+    __syncthreads();
+  }
+  auto eq_unit = [&](C (&e)[Q]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) e[q] = s_eq[q];
+  };
 
   C pre[Q];  // pulled populations of this lane's step-one cell on the next plane
-  bool pre_solid = false;
+  int pre_mask = 0;  // its solid byte, compared only where step one uses it
   auto prefetch = [&](int p) {
     if (!h1) return;
     const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
-    gather<L, S>(pre, fin, a, gx, gy, pz);
+    // the mask byte first: the loads complete in order, so consuming it does not
+    // wait for the population loads behind it
     if constexpr (kAll) {
-      if (a.solid != nullptr)
-        pre_solid = a.solid[(uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx)] != 0;
+      if (a.solid != nullptr) pre_mask = a.solid[(uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx)];
     }
+    gather<L, S>(pre, fin, a, gxn, gy, pz);
   };
-  auto step_one = [&](int p, int slot, bool owned_plane) {
+  auto step_one = [&](int p, int slot) {
     if (!h1) return;
     const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
     const uint32_t cell = (uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx);
-    double dx = 0.0, dy = 0.0, dz = 0.0;
-    bool solid = false;
-    if (kAll && edge_bc) {
-      const Contrib r = step_one_edge<L, S, COLL, FEAT>(a, gx, gy, pz, cell, s + size_t(slot) * Q * EC + t, EC);
-      dx = r.fx;
-      dy = r.fy;
-      dz = r.fz;
-      solid = r.solid;
-    } else {
-      C f[Q];
+    C f[Q];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) f[q] = pre[q];
-      solid = pre_solid;
-      auto pulled = [&](int q, int px, int py, int qz) { return pull_one<ST, L>(fin, a, q, px, py, qz); };
-      update_cell<L, S, COLL, true, FEAT, false>(f, a, gx, gy, pz, cell, solid, dx, dy, dz, pulled);
+    for (int q = 0; q < Q; ++q) f[q] = pre[q];
+    auto unused = [](int, int, int, int) { return C(0); };
+    const bool solid = pre_mask != 0;
+    update_cell<L, S, COLL, true, FEAT, BCN_IN_F, BCK>(f, a, gx, gy, pz, cell, solid, nb_edge, unused_force,
+                                                       unused_force, unused_force, unused, eq_unit);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) s[(slot * Q + q) * EC + t] = ST::template put<L>(f[q], q);
-    }
-    if (own1 && owned_plane) {
-      fx += dx;
-      fy += dy;
-      fz += dz;
-      any_solid |= solid;
-    }
+    for (int q = 0; q < Q; ++q) s[(slot * Q + q) * EC + t] = ST::template put<L>(f[q], q);
+    if constexpr (kAll) s_solid[slot * EC + t] = solid ? 1 : 0;
   };
 
   // iteration k computes step one on plane z0 + k + 1 (slot (k + 2) % 3) and, from
@@ -432,7 +498,7 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   for (int k = -2; z0 + k < zend; ++k) {
     const int z = z0 + k;
     const int p1 = z + 1;
-    step_one(p1, (k + 2) % 3, p1 >= z0 && p1 < zend);
+    step_one(p1, (k + 2) % 3);
     if (p1 < zend) prefetch(p1 + 1);  // the next iteration's step-one plane
     lds_barrier();
     if (k >= 0 && st) {
@@ -448,20 +514,15 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
       for (int q = 0; q < Q; ++q) f[q] = lds_at(q, ux_, uy_, z);
       const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(uy_)) * uint32_t(nx) + uint32_t(ux_);
       bool solid = false;
-      if constexpr (kAll) {
-        if (a.solid != nullptr) solid = a.solid[cell] != 0;
-      }
-      update_cell<L, S, COLL, true, FEAT, true>(f, a, ux_, uy_, z, cell, solid, fx, fy, fz, lds_at);
-      any_solid |= solid;
+      if constexpr (kAll) solid = s_solid[((k + 1) % 3) * EC + (oy + 1) * EX + (ox + 1)] != 0;
+      update_cell<L, S, COLL, true, FEAT, BCN_PULL, BCK>(f, a, ux_, uy_, z, cell, solid, false, unused_force,
+                                                         unused_force, unused_force, lds_at, eq_unit);
       T* fout = static_cast<T*>(a.fout);
       const uint32_t off = uint32_t(uy_) * uint32_t(a.osy) + uint32_t(ux_);
 #pragma unroll
       for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
     }
     lds_barrier();
-  }
-  if constexpr (kAll) {
-    if (a.force != nullptr) force_flush(a.force, fx, fy, fz, any_solid);
   }
 }
 
@@ -476,9 +537,9 @@ constexpr int tb_threads() {
   return ((TX + 2) * (TY + 2) + 63) / 64 * 64;
 }
 
-template <class L, int S, int COLL, int FEAT, int TX, int TY>
+template <class L, int S, int COLL, int FEAT, int BCK, int TX, int TY>
 __global__ __launch_bounds__((tb_threads<TX, TY>())) void k_lbm2(const KArgs<typename Store<S>::C> a, int zc) {
-  lbm2_body<L, S, COLL, FEAT, TX, TY>(a, zc);
+  lbm2_body<L, S, COLL, FEAT, BCK, TX, TY>(a, zc);
 }
 
 // The learned-closure variants hold the matrix-core tiles next to the Q
@@ -834,14 +895,19 @@ int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb
 
 // ---- two steps per launch ----------------------------------------------------
 
-// tile of the two-step kernel: 64 x 6 output cells; 32 x 8 where three planes
-// of 64 x 6 would not fit the 160 KB of LDS (D3Q27, 4-byte storage)
+// Tile of the two-step kernel: 64 x 6 output cells (9 waves, 1.38x halo reads)
+// where three planes fit the 160 KB of LDS, else 32 x 8 (6 waves; D3Q27 with
+// 4-byte storage).
 template <class L, int S>
 struct Tile2 {
   static constexpr bool wide = 3 * L::Q * 66 * 8 * sizeof(typename Store<S>::T) <= 160 * 1024;
   static constexpr int TX = wide ? 64 : 32;
   static constexpr int TY = wide ? 6 : 8;
 };
+
+inline int tile2_width(int lattice, int store) {
+  return lattice == TLBM_D3Q27 && store == TLBM_STORE_F32 ? 32 : 64;
+}
 
 template <class L, int S>
 constexpr bool step2_store() {
@@ -864,20 +930,40 @@ inline const char* step2_reason(int lattice, int store, const tlbm_collision* co
   if (coll->tau_mode == TLBM_TAU_MLP) return "the learned closure runs in the one-step kernel";
   if (bc != nullptr && (bc->kind == TLBM_BC_CHANNEL_X || bc->kind == TLBM_BC_FAR_FIELD)) {
     // the outlet copies x = nx-2, whose pulls reach x = nx-3: same tile as x = nx-1
-    const int tx = lattice == TLBM_D3Q27 && store == TLBM_STORE_F32 ? 32 : 64;
-    if (nx < 4 || (nx - 1) % tx < 2) return "the outlet column would straddle two tiles";
+    if (nx < 4 || (nx - 1) % tile2_width(lattice, store) < 2) return "the outlet column would straddle two tiles";
   }
   return nullptr;
 }
 
-template <class L, int S, int COLL, int FEAT>
+template <class L, int S, int COLL, int FEAT, int BCK>
 int launch_lbm2(const KArgs<typename Store<S>::C>& a, hipStream_t s) {
   constexpr int TX = Tile2<L, S>::TX, TY = Tile2<L, S>::TY;
   const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY;
   const int zc = step2_zc(gx * gy, a.nz);
   const dim3 grid(gx, gy, (a.nz + zc - 1) / zc);
-  hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, TX, TY>), grid, dim3(tb_threads<TX, TY>()), 0, s, a, zc);
+  hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, BCK, TX, TY>), grid, dim3(tb_threads<TX, TY>()), 0, s, a, zc);
   return check_launch("k_lbm2");
+}
+
+// D3Q19 builds one two-step kernel per boundary kind: each kind's uniform state
+// then fits the scalar registers (all kinds in one kernel spilled ~60 SGPRs into
+// VGPR lanes, doubling the VALU work of the z loop); D3Q27 reads the kind at run time.
+template <class L, int S, int COLL>
+int launch_lbm2_bc(const KArgs<typename Store<S>::C>& a, hipStream_t s) {
+  if constexpr (L::Q == 19) {
+    switch (a.bc_kind) {
+      case TLBM_BC_CHANNEL_X:
+        return launch_lbm2<L, S, COLL, FEAT_BC, TLBM_BC_CHANNEL_X>(a, s);
+      case TLBM_BC_LID_Y:
+        return launch_lbm2<L, S, COLL, FEAT_BC, TLBM_BC_LID_Y>(a, s);
+      case TLBM_BC_FAR_FIELD:
+        return launch_lbm2<L, S, COLL, FEAT_BC, TLBM_BC_FAR_FIELD>(a, s);
+      default:
+        return launch_lbm2<L, S, COLL, FEAT_BC, TLBM_BC_NONE>(a, s);
+    }
+  } else {
+    return launch_lbm2<L, S, COLL, FEAT_BC, BCK_RUNTIME>(a, s);
+  }
 }
 
 template <class L, int COLL>
@@ -927,8 +1013,8 @@ int run_coll(const Call& c) {
     }
     if (c.steps == 2) {
       if constexpr (step2_store<L, S>() && COLL != TLBM_COLL_NONE) {
-        if (c.feat) return launch_lbm2<L, S, COLL, FEAT_ALL>(a, c.s);
-        return launch_lbm2<L, S, COLL, FEAT_NONE>(a, c.s);
+        if (c.feat) return launch_lbm2_bc<L, S, COLL>(a, c.s);
+        return launch_lbm2<L, S, COLL, FEAT_NONE, TLBM_BC_NONE>(a, c.s);
       } else {
         return fail(TLBM_ERR_UNSUPPORTED, "two-step launch not built for this lattice / storage");
       }
@@ -1100,9 +1186,10 @@ int tlbm_step2(int lattice, int store, const void* f_in, const int64_t* in_strid
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
   if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
   if (const char* why = step2_reason(lattice, store, coll, bc, nx)) return fail(TLBM_ERR_UNSUPPORTED, "%s", why);
-  const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
-  if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
-  Call c{lattice, store, coll->kind, true, feat, 2, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
+  if (force != nullptr)
+    return fail(TLBM_ERR_UNSUPPORTED, "the two-step launch takes no force buffer; sample the force with tlbm_step");
+  const bool feat = solid != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
+  Call c{lattice, store, coll->kind, true, feat, 2, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, nullptr,
          (hipStream_t)stream};
   return run(c);
 }

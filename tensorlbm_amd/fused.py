@@ -237,10 +237,13 @@ class FusedSolver3D:
     viscosity, evaluated inside the step (``C_s`` then sets the strain's time
     scale: the Smagorinsky τ of that constant, or τ itself for ``C_s=0``).
 
-    ``run()`` advances pairs of steps with one launch each where the library
-    builds that case (``temporal_blocking=True``, the default): the same
+    ``run()`` advances pairs of steps with one launch each when
+    ``temporal_blocking`` is on and the library builds the case: the same
     results as single steps, one HBM round trip of the populations per two
-    steps.  ``step2()`` is one such pair.
+    steps.  ``step2()`` is one such pair.  ``None`` (default) turns it on for
+    D3Q19 with fp32 storage, where the step is bound by HBM; fp16 storage and
+    D3Q27 are bound by instruction issue on MI355X, and the halo the two-step
+    tile recomputes makes them slower.
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
@@ -248,7 +251,7 @@ class FusedSolver3D:
                  collision: str = "BGK", rates: Sequence[float] | None = None, deviation: bool = False,
                  C_s: float = 0.0, delta: float = 1.0, solid: torch.Tensor | None = None,
                  boundary: str | None = None, u_bc=(0.0, 0.0, 0.0), layout: str = "padded",
-                 closure=None, bc_config: dict | None = None, temporal_blocking: bool = True,
+                 closure=None, bc_config: dict | None = None, temporal_blocking: bool | None = None,
                  **_tiles) -> None:
         if precision is not None:
             policy = PrecisionPolicy.parse(precision)
@@ -312,6 +315,8 @@ class FusedSolver3D:
         self._buf_a: torch.Tensor | None = None
         self._buf_b: torch.Tensor | None = None
         self._report_step = 0
+        if temporal_blocking is None:
+            temporal_blocking = lat == "D3Q19" and self._store == nat.STORE_F32
         self._two_step = bool(temporal_blocking) and lat != "D2Q9" and bool(
             nat.lib().tlbm_step2_supported(self._Q, self._store, self._coll, self._bc, self.nx))
 
@@ -349,8 +354,9 @@ class FusedSolver3D:
                             deviation=self.deviation, what="FusedSolver3D.step")
 
     def step2(self, f: torch.Tensor, *, force: torch.Tensor | None = None) -> torch.Tensor:
-        """Advance two steps (one launch when ``two_step``); ``force`` receives both steps' samples."""
-        if not self._two_step:
+        """Advance two steps (one launch when ``two_step``); ``force`` receives both steps' samples
+        (the Ladd force is sampled by the one-step kernel, so passing it takes two launches)."""
+        if not self._two_step or force is not None:
             return self.step(self.step(f, force=force), force=force)
         self._check_input(f)
         out = self._target(f, force)

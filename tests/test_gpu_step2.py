@@ -6,10 +6,11 @@ two single steps up to rounding: the two kernels inline that code into
 different contexts and the compiler may fuse multiply-adds differently, which
 moves a value by an ulp of fp32 (or, rounded into 16-bit storage, by one ulp of
 the stored type).  The tests require that bar everywhere and identical values
-almost everywhere (BGK / MRT / CM / cumulant measure bit-identical here).  Grids are ragged against
-the 64 x 6 (D3Q27 fp32: 32 x 8) tiles and the z blocking, and one is smaller
-than a tile so the halo wraps more than once.  The integer canary pins the LDS
-addressing (including the z slots and bounce-back) to the oracle's pull.
+almost everywhere (measured bit-identical here).  Grids are ragged against the
+64 x 6 tiles (32 x 8 for D3Q27 fp32) and the z
+blocking, and one is smaller than a tile so the halo wraps more than once.  The
+integer canary pins the LDS addressing (including the z slots and bounce-back)
+to the oracle's pull.
 """
 
 from __future__ import annotations
@@ -33,7 +34,8 @@ def _solvers(q, shape, tau, store="fp32", **kw):
     nz, ny, nx = shape
     one = tl.FusedSolver3D(nz, ny, nx, tau, "cuda:0", dtype, lattice=LAT[q], deviation=dev_store,
                            temporal_blocking=False, **kw)
-    two = tl.FusedSolver3D(nz, ny, nx, tau, "cuda:0", dtype, lattice=LAT[q], deviation=dev_store, **kw)
+    two = tl.FusedSolver3D(nz, ny, nx, tau, "cuda:0", dtype, lattice=LAT[q], deviation=dev_store,
+                           temporal_blocking=True, **kw)
     return one, two
 
 
@@ -100,16 +102,49 @@ def test_grids_smaller_than_a_tile(shape):
 
 
 @pytest.mark.parametrize("store", ["fp32", "fp16dev"])
-def test_sphere_channel_with_force(store):
+def test_sphere_channel(store):
     nz, ny, nx = 18, 22, 70  # (nx - 1) % 64 = 5: outlet column pulls stay in the last tile
     solid = torch.from_numpy(ref.sphere_mask(nx, ny, nz, 20.0, 11.0, 9.0, 5.5)).to(torch.int8)
     one, two = _solvers(19, (nz, ny, nx), 0.56, store, solid=solid, boundary="channel_x", u_bc=(0.05, 0.0, 0.0),
                         C_s=0.17)
     assert two.two_step
-    a, b, fa, fb = _pair(one, two, _initial(19, (nz, ny, nx), store, seed=11, u_amp=0.02), pairs=3, force=True)
+    a, b, _, _ = _pair(one, two, _initial(19, (nz, ny, nx), store, seed=11, u_amp=0.02), pairs=3)
+    _assert_equal(a, b)
+
+
+def test_channel_walls_touching_inlet_and_outlet():
+    """Solid cells on the inlet and outlet planes (channel walls): the step-one inlet / outlet
+    lanes gather their neighbouring column, the wall cells among them bounce back."""
+    nz, ny, nx = 10, 12, 70
+    wall = np.zeros((nz, ny, nx), dtype=bool)
+    wall[:, 0, :] = wall[:, -1, :] = True
+    wall[3:6, 4:7, 30:34] = True
+    one, two = _solvers(19, (nz, ny, nx), 0.6, "fp32", solid=torch.from_numpy(wall), boundary="channel_x",
+                        u_bc=(0.04, 0.0, 0.0))
+    assert two.two_step
+    a, b, _, _ = _pair(one, two, _initial(19, (nz, ny, nx), "fp32", seed=18, u_amp=0.02), pairs=3)
+    _assert_equal(a, b)
+
+
+def test_force_takes_the_one_step_kernel():
+    """tlbm_step2 declines a force buffer; FusedSolver3D.step2(force=) takes two single steps."""
+    from tensorlbm_amd import _native as nat
+
+    nz, ny, nx = 12, 12, 70
+    solid = torch.from_numpy(ref.sphere_mask(nx, ny, nz, 20.0, 6.0, 6.0, 3.5)).to(torch.int8)
+    one, two = _solvers(19, (nz, ny, nx), 0.6, "fp32", solid=solid, boundary="channel_x", u_bc=(0.05, 0.0, 0.0))
+    f0 = _initial(19, (nz, ny, nx), "fp32", seed=19, u_amp=0.02)
+    a, b, fa, fb = _pair(one, two, f0, pairs=2, force=True)
     _assert_equal(a, b)
     assert abs(fa[0].item()) > 0
-    np.testing.assert_allclose(fb.cpu().numpy(), fa.cpu().numpy(), rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(fb.cpu().numpy(), fa.cpu().numpy(), rtol=1e-12, atol=1e-14)
+    out = torch.empty_like(f0)
+    acc = torch.zeros(3, dtype=torch.float64, device="cuda:0")
+    coll = nat.make_collision(nat.COLL_BGK, 0.6)
+    rc = nat.lib().tlbm_step2(19, nat.STORE_F32, f0.data_ptr(), None, out.data_ptr(), None, nz, ny, nx, coll,
+                              solid.to("cuda:0").data_ptr(), None, acc.data_ptr(), None)
+    assert rc == nat.ERR_UNSUPPORTED
+    assert b"force" in nat.lib().tlbm_last_error()
 
 
 def test_d3q27_tile_channel():
@@ -179,6 +214,14 @@ def test_two_fused_bgk_steps_match_oracle():
     got = two.step2(torch.from_numpy(f0).to("cuda:0")).cpu().numpy()
     want = ref.fused_step(ref.fused_step(f0.astype(np.float64), 0.7), 0.7)
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
+
+
+def test_default_policy():
+    """Two steps per launch by default only where the step is HBM-bound (D3Q19, fp32 storage)."""
+    assert tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32).two_step
+    assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float16, deviation=True).two_step
+    assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, lattice="D3Q27").two_step
+    assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float64).two_step
 
 
 def test_unsupported_arguments_raise():
