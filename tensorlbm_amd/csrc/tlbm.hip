@@ -396,6 +396,17 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   }
 }
 
+// Workgroup w of a 1-D grid of n runs on XCD w % 8 (round-robin dispatch over the
+// 8 dies, each with its own L2).  Give XCD c the contiguous tile range c*n/8 ..
+// so that tiles sharing halo rows (x and y neighbours, x fastest) run at the same
+// time on the same die and meet their halo lines in that die's L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t w, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds;  // XCDs below r take q + 1 tiles
+  const uint32_t c = w % kXcds, slot = w / kXcds;
+  return c < r ? c * (q + 1) + slot : r * (q + 1) + (c - r) * q + slot;
+}
+
 // LDS writes complete, then the workgroup barrier.  Unlike __syncthreads() this
 // does not wait for outstanding global loads, so the next plane's prefetch
 // stays in flight across the barrier.
@@ -425,7 +436,9 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   __shared__ C s_eq[kAll ? Q : 1];
   const int t = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int x0 = int(blockIdx.x) * TX, y0 = int(blockIdx.y) * TY, z0 = int(blockIdx.z) * zc;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t ntx = uint32_t((nx + TX - 1) / TX), nty = uint32_t((ny + TY - 1) / TY);
+  const int x0 = int(tile % ntx) * TX, y0 = int((tile / ntx) % nty) * TY, z0 = int(tile / (ntx * nty)) * zc;
   const int zend = z0 + zc < nz ? z0 + zc : nz;
   const T* fin = static_cast<const T*>(a.fin);
   // step-one cell of this lane (tile-local lx, ly; global gx, gy)
@@ -940,8 +953,8 @@ int launch_lbm2(const KArgs<typename Store<S>::C>& a, hipStream_t s) {
   constexpr int TX = Tile2<L, S>::TX, TY = Tile2<L, S>::TY;
   const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY;
   const int zc = step2_zc(gx * gy, a.nz);
-  const dim3 grid(gx, gy, (a.nz + zc - 1) / zc);
-  hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, BCK, TX, TY>), grid, dim3(tb_threads<TX, TY>()), 0, s, a, zc);
+  const uint32_t tiles = uint32_t(gx) * uint32_t(gy) * uint32_t((a.nz + zc - 1) / zc);
+  hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, BCK, TX, TY>), dim3(tiles), dim3(tb_threads<TX, TY>()), 0, s, a, zc);
   return check_launch("k_lbm2");
 }
 

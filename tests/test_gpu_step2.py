@@ -217,8 +217,9 @@ def test_two_fused_bgk_steps_match_oracle():
 
 
 def test_default_policy():
-    """Two steps per launch by default only where the step is HBM-bound (D3Q19, fp32 storage)."""
+    """Two steps per launch by default only for D3Q19 BGK with fp32 storage."""
     assert tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32).two_step
+    assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, collision="MRT").two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float16, deviation=True).two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, lattice="D3Q27").two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float64).two_step
