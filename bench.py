@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--no-forms", action="store_true", help="only the headline form")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--copy-probe", action="store_true", help="also time a plain device copy of the same bytes")
+    p.add_argument("--skip", default="", help="comma-separated forms to leave out: cavities, dhit")
     return p.parse_args()
 
 
@@ -268,8 +269,11 @@ def main():
         forms["bgk_fp16dev_mlp_channel_sphere"] = form(torch.float16, deviation=True, mlp=True)
         forms["bgk_fp32_mlp_channel_sphere"] = form(torch.float32, mlp=True)
         forms["fp16dev_mlp_speedup_vs_fp32"] = round(forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"] / head["mlups"], 3)
-        forms["cavities_64"] = cavities_form(torch, dev)
-        forms["dhit_d3q27_geier"] = dhit_forms(torch, dev, n)
+        skip = {x.strip() for x in args.skip.split(",") if x.strip()}
+        if "cavities" not in skip:
+            forms["cavities_64"] = cavities_form(torch, dev)
+        if "dhit" not in skip:
+            forms["dhit_d3q27_geier"] = dhit_forms(torch, dev, n)
     if args.copy_probe:
         src = f0.clone()
         dst = torch.empty_like(src)
@@ -316,6 +320,8 @@ def main():
             "unit": "GB/s",
             "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(n, per),
+            "traffic_GBs": (round(measured_traffic(n, per) / (head["ms_per_launch"] * 1e-3) / 1e9, 1)
+                            if measured_traffic(n, per) else None),
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; " + str(
                 PMC_SUMMARY[per].relative_to(ROOT)),
             "algorithmic_bytes_per_launch": head["bytes_per_cell"] * cells * per,

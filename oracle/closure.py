@@ -107,14 +107,15 @@ def split_fp16(a):
 
 def forward_mfma(vec, x, mean, inv_std):
     """ν_t as the fused collision evaluates it on the matrix cores: fp16 W1 and W2, normalised
-    inputs as two-term fp16 splits, layer-1 activations relu(z1) rounded once to fp16, wide
-    accumulation, fp32 output layer."""
+    inputs and the layer-1 bias as two-term fp16 splits, layer-1 activations relu(z1) rounded
+    once to fp16, wide accumulation, fp32 output layer."""
     p = unpack(np.asarray(vec, dtype=np.float64))
     f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
     xh = f32((f32(x) - f32(mean)) * f32(inv_std))
     xhi, xlo = split_fp16(xh)
     w1 = fp16(p["w1"])
-    z1 = f32(xhi @ w1.T + xlo @ w1.T + f32(p["b1"]))
+    b1h, b1l = split_fp16(p["b1"])  # the bias enters the tile as hi + lo against a constant 1
+    z1 = f32(xhi @ w1.T + xlo @ w1.T + (b1h + b1l))
     h1 = fp16(np.maximum(z1, 0.0))
     w2 = fp16(p["w2"])
     z2 = f32(h1 @ w2.T + f32(p["b2"]))

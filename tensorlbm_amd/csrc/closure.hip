@@ -37,7 +37,13 @@ __global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ p, Pa
   const int l = threadIdx.x;
   const int h = l >> 5, i = l & 31;
   for (int e = 0; e < 8; ++e) {
-    out->a1[l][e] = e < NIN ? f16_bits(p[OW1 + i * NIN + e]) : uint16_t(0);
+    if (e < NIN) {
+      out->a1[l][e] = f16_bits(p[OW1 + i * NIN + e]);
+    } else {  // bias at K 7 (hi, lanes < 32) and K 15 (lo, lanes >= 32)
+      const float b = p[OB1 + i];
+      const float bh = static_cast<float>(static_cast<_Float16>(b));
+      out->a1[l][e] = f16_bits(h == 0 ? b : b - bh);
+    }
     for (int s = 0; s < 2; ++s) out->a2[s][l][e] = f16_bits(p[OW2 + i * NH + w2_col(s, h, e)]);
   }
   if (l < 32) {

@@ -10,9 +10,10 @@
 //   C[i][j] : lane j + 32*((i/4)%2), element (i%4) + 4*(i/8)
 // Hidden units are rows (i) and cells are columns (j).  The normalised inputs
 // enter as two-term fp16 splits v = hi + lo (hi = fp16(v), lo = fp16(v - hi),
-// ~22 bits); layer 1 puts the hi parts at K 0..7 and the lo parts at K 8..15 of
+// ~22 bits); layer 1 puts the hi parts at K 0..6 and the lo parts at K 8..14 of
 // one tile per 32-cell block (one cross-half lane exchange supplies the partner
-// lane's half).  The layer-1 C tile becomes the layer-2 B operand in registers:
+// lane's half), and the bias, split the same way, at K 7 and 15 against a
+// constant 1, so the accumulator starts from zero.  The layer-1 C tile becomes the layer-2 B operand in registers:
 // relu(c) rounded to fp16 (11 bits, packed convert + packed max, one
 // instruction per value), W2's columns permuted at packing time to match the
 // C layout.  The output layer is 16 local fp32 FMAs plus one cross-half
@@ -43,9 +44,10 @@ constexpr float BETA = 10.0f;
 
 // Matrix-core operand blob, built once per weight update by tlbm_mlp_pack.
 struct Packed {
-  uint16_t a1[64][8];     // layer-1 A (fp16 bits): W1 row (lane % 32), inputs 0..6 (both K halves)
+  uint16_t a1[64][8];     // layer-1 A (fp16 bits): W1 row (lane % 32), inputs 0..6 (both K halves),
+                          // element 7: b1 hi (lanes < 32) / lo (lanes >= 32)
   uint16_t a2[2][64][8];  // layer-2 A (fp16 bits), K-steps 0 and 1, columns in C-layout order
-  float b1c[2][16];       // layer-1 bias, C layout per lane half
+  float b1c[2][16];       // layer-1 bias, C layout per lane half (reference copy; the tile takes it via K)
   float b2c[2][16];       // layer-2 bias
   float w3c[2][16];       // output weights
   float b3;
@@ -127,7 +129,8 @@ __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
     hi[e] = static_cast<_Float16>(v);
     lo[e] = static_cast<_Float16>(v - static_cast<float>(hi[e]));
   }
-  hi[7] = lo[7] = _Float16(0);
+  // K 7 and 15 carry the layer-1 bias (hi and lo parts in A) against a constant 1
+  hi[7] = lo[7] = _Float16(1);
   // block 0 (cells of lanes 0..31): K 0..7 = own hi on lanes < 32, K 8..15 = the
   // partner's lo on lanes >= 32; block 1 (cells of lanes 32..63) the other way round
   const f16x8 got = xor32(h == 0 ? lo : hi);
@@ -137,14 +140,10 @@ __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
   const f16x8 a20 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
   const f16x8 a21 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
   // one 32-cell block at a time keeps a single accumulator tile pair live
-  f32x16 c;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) c[r] = P->b1c[h][r];
-  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, c, 0, 0, 0);
+  const f32x16 zero = {};
+  f32x16 c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, zero, 0, 0, 0);
   const float p0 = block_out(P, h, c, a20, a21);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) c[r] = P->b1c[h][r];
-  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, zero, 0, 0, 0);
   const float p1 = block_out(P, h, c, a20, a21);
   return softplus((h == 0 ? p0 : p1) + P->b3);
 }

@@ -123,6 +123,10 @@ extern __shared__ __align__(16) unsigned char dyn_lds[];
 #define TLBM_MLP_WAVES 4
 #endif
 
+// 1/x by the hardware reciprocal (1 ulp) in fp32; exact division in fp64
+__device__ __forceinline__ float approx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double approx_rcp(double x) { return 1.0 / x; }
+
 // Equilibrium of the boundary velocity u (a launch constant) computed where it
 // is used.  The empty asm makes the compiler treat the velocity as produced at
 // this point: otherwise it hoists the Q equilibrium values out of the z loop of
@@ -254,19 +258,20 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       // Smagorinsky tau as reference time scale, S = -Pi / (2 rho cs^2 tau_ref)
       C pi[6];
       neq_stress_moments<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
-      const C om_ref = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
-      const C sc = C(-1.5) * om_ref / mc.rho_safe;
+      const C te_ref = smagorinsky_tau(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
+      // features feed an fp16 network: hardware reciprocal / square root (1 ulp)
+      const C sc = C(-1.5) * approx_rcp(te_ref * mc.rho_safe);
       float xs[7];
 #pragma unroll
       for (int k = 0; k < 6; ++k) xs[k] = float(sc * pi[k]);
-      xs[6] = sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
-                            2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
+      xs[6] = __builtin_amdgcn_sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
+                                             2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
       // operands straight from the (cache-resident, 3.5 KB) blob: no staging barrier
       const float nu = mlp::nu_t(static_cast<const mlp::Packed*>(a.mlp), xs);
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
-      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, C(1) / te);
+      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, approx_rcp(te));
     } else if constexpr (COLL == TLBM_COLL_BGK) {
       if (need_pi) {
         C feq[Q], fneq[Q], pi[6];
