@@ -350,7 +350,7 @@ __device__ __forceinline__ void force_flush(double* force, double fx, double fy,
   }
 }
 
-template <class L, int S, int COLL, bool STREAM, int FEAT>
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
 __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   using ST = Store<S>;
   using T = typename ST::T;
@@ -381,15 +381,18 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(ld_off(fin + int64_t(q) * a.isq + int64_t(z) * a.isz, off), q);
   }
-  bool solid = false;
+  // (issued after the gather: measured faster here than ahead of it, unlike the
+  // two-step kernel's prefetch)
+  int mask = 0;
   if constexpr (kAll) {
-    if (a.solid != nullptr) solid = active && a.solid[cell] != 0;
+    if (a.solid != nullptr && active) mask = a.solid[cell];
   }
+  const bool solid = mask != 0;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
   auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
-  update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK_RUNTIME>(f, a, xx, yy, z, cell, solid, false, fx, fy, fz,
-                                                                pulled, eq_unit);
+  update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK>(f, a, xx, yy, z, cell, solid, false, fx, fy, fz, pulled,
+                                                        eq_unit);
   if (active) {
     T* fout = static_cast<T*>(a.fout);
     const uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
@@ -544,9 +547,9 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   }
 }
 
-template <class L, int S, int COLL, bool STREAM, int FEAT>
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
 __global__ __launch_bounds__(256) void k_lbm(const KArgs<typename Store<S>::C> a) {
-  lbm_body<L, S, COLL, STREAM, FEAT>(a);
+  lbm_body<L, S, COLL, STREAM, FEAT, BCK>(a);
 }
 
 // two-step tile: TX x TY output cells, (TX+2)(TY+2) lanes rounded to whole waves
@@ -563,10 +566,10 @@ __global__ __launch_bounds__((tb_threads<TX, TY>())) void k_lbm2(const KArgs<typ
 // The learned-closure variants hold the matrix-core tiles next to the Q
 // populations; the occupancy floor keeps enough waves in flight to cover HBM
 // latency.
-template <class L, int S, int COLL, bool STREAM, int FEAT>
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TLBM_MLP_WAVES))) void k_lbm_mlp(
     const KArgs<typename Store<S>::C> a) {
-  lbm_body<L, S, COLL, STREAM, FEAT>(a);
+  lbm_body<L, S, COLL, STREAM, FEAT, BCK>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -901,14 +904,32 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
 
 // ---- instantiation table ----------------------------------------------------
 
-template <class L, int S, int COLL, bool STREAM, int FEAT>
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK = BCK_RUNTIME>
 int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
   if constexpr ((FEAT & FEAT_MLP) != 0) {
-    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
+    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT, BCK>), dim3(nb), dim3(bx, by), 0, s, a);
   } else {
-    hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT>), dim3(nb), dim3(bx, by), 0, s, a);
+    hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT, BCK>), dim3(nb), dim3(bx, by), 0, s, a);
   }
   return check_launch("k_lbm");
+}
+
+// D3Q19 BGK fused steps without a force buffer (the benchmarked and closure
+// forms) get one kernel per boundary kind: the uniform state of all kinds and
+// of the force sample together spills scalar registers into VGPR lanes, which
+// costs the instruction-bound 16-bit and closure forms directly.
+template <class L, int S, int COLL, int FEAT>
+int launch_lbm_kind(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
+  switch (a.bc_kind) {
+    case TLBM_BC_CHANNEL_X:
+      return launch_lbm<L, S, COLL, true, FEAT, TLBM_BC_CHANNEL_X>(a, bx, by, nb, s);
+    case TLBM_BC_LID_Y:
+      return launch_lbm<L, S, COLL, true, FEAT, TLBM_BC_LID_Y>(a, bx, by, nb, s);
+    case TLBM_BC_FAR_FIELD:
+      return launch_lbm<L, S, COLL, true, FEAT, TLBM_BC_FAR_FIELD>(a, bx, by, nb, s);
+    default:
+      return launch_lbm<L, S, COLL, true, FEAT, TLBM_BC_NONE>(a, bx, by, nb, s);
+  }
 }
 
 // ---- two steps per launch ----------------------------------------------------
@@ -1023,6 +1044,7 @@ int run_coll(const Call& c) {
     if (c.coll != nullptr && c.coll->tau_mode == TLBM_TAU_MLP) {
       if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK) {
         if (!c.stream_) return fail(TLBM_ERR_UNSUPPORTED, "the learned closure runs in the fused step");
+        if (c.feat && c.force == nullptr) return launch_lbm_kind<L, S, COLL, FEAT_MLP | FEAT_BC>(a, bx, by, nb, c.s);
         if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_MLP_ALL>(a, bx, by, nb, c.s);
         return launch_lbm<L, S, COLL, true, FEAT_MLP>(a, bx, by, nb, c.s);
       } else {
@@ -1038,6 +1060,9 @@ int run_coll(const Call& c) {
       }
     }
     if (c.stream_) {
+      if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK) {
+        if (c.feat && c.force == nullptr) return launch_lbm_kind<L, S, COLL, FEAT_BC>(a, bx, by, nb, c.s);
+      }
       if (c.feat) return launch_lbm<L, S, COLL, true, FEAT_ALL>(a, bx, by, nb, c.s);
       return launch_lbm<L, S, COLL, true, FEAT_NONE>(a, bx, by, nb, c.s);
     }
