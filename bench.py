@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--copy-probe", action="store_true", help="also time a plain device copy of the same bytes")
     p.add_argument("--skip", default="", help="comma-separated forms to leave out: cavities, dhit")
+    p.add_argument("--one-step", action="store_true", help="headline form with one step per launch (k_lbm)")
     return p.parse_args()
 
 
@@ -251,7 +252,7 @@ def main():
                 "bytes_per_cell": bpc, "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "finite": ok, "wall_s": round(wall, 3)}
 
-    head = form(torch.float32)
+    head = form(torch.float32, two_step=False if args.one_step else None)
     forms = {}
     if not args.no_forms:
         forms["bgk_fp32_channel_sphere_one_step_per_launch"] = form(torch.float32, two_step=False)
