@@ -222,12 +222,13 @@ def _r2(pred: torch.Tensor, y: torch.Tensor) -> float:
 
 def train_eddy_viscosity_model_3d(features: torch.Tensor, targets: torch.Tensor, config: TrainConfig3D | None = None,
                                   *, device: str | torch.device = "cuda:0",
-                                  model: EddyViscosityMLP3D | None = None) -> tuple[EddyViscosityMLP3D, dict]:
+                                  model: EddyViscosityMLP3D | None = None, log=None) -> tuple[EddyViscosityMLP3D, dict]:
     """Fit an :class:`EddyViscosityMLP3D` to (features (N, 7), ν targets (N,)) on the GPU.
 
     Follows ``train_eddy_viscosity_model`` (ai/train.py:310-464): seeded train/validation split,
     feature mean / std from the training part, shuffled minibatches from a CPU generator, MSE,
-    gradient-norm clipping, Adam, and the best-validation parameters kept.
+    gradient-norm clipping, Adam, and the best-validation parameters kept.  ``log(msg)``, when
+    given, receives one line per epoch.
     """
     cfg = config or TrainConfig3D()
     device = torch.device(device) if model is None else model.device
@@ -265,6 +266,8 @@ def train_eddy_viscosity_model_3d(features: torch.Tensor, targets: torch.Tensor,
         met = {"epoch": epoch, "train_mse": float(ep_loss) / n_tr, "val_mse": val_mse,
                "val_mae": float(torch.mean(torch.abs(pred - y_val))), "val_r2": _r2(pred, y_val)}
         history.append(met)
+        if log is not None:
+            log(f"epoch {epoch}: train mse {met['train_mse']:.3e}, val mse {val_mse:.3e}, val R2 {met['val_r2']:.3f}")
         if val_mse < best[0] - 1e-30:
             best = (val_mse, model.params.clone(), epoch)
             stale = 0
@@ -356,8 +359,7 @@ def run_dns_reference_3d(n_les: int, tau_dns: float, n_steps_les: int, sample_ev
     solver = FusedSolver3D(n, n, n, tau_dns, device, torch.float32, layout="contiguous")
     snaps = [coarsen_moments(f, r)]
     for k in range(1, int(n_steps_les) + 1):
-        for _ in range(r):
-            f = solver.step(f)
+        f, _ = solver.run(f, r)  # two fine steps per launch where the solver builds that case
         if sample_every > 0 and k % sample_every == 0:
             snaps.append(coarsen_moments(f, r))
     return snaps

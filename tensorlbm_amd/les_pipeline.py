@@ -12,7 +12,8 @@ block-averages it, trains ``EddyViscosityMLP`` and validates an LES that uses it
 4. per snapshot: velocity rel-L2 and kinetic-energy error against the filtered DNS.
 
 ``python -m tensorlbm_amd.les_pipeline --n-les 192 --dns-scale 3`` is the 576³ → 192³ case
-(about 29 GB for the DNS); smaller sizes run in seconds.
+(about 29 GB for the DNS); smaller sizes run in seconds.  ``--k0`` moves the spectrum peak of
+the initial field; training uses at most ``max_samples`` (a seeded random subset).
 """
 
 from __future__ import annotations
@@ -40,17 +41,16 @@ def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32,
     s = FusedSolver3D(n, n, n, tau, device, dtype, deviation=dev_store, closure=closure, C_s=C_s)
     errs, kes = [], []
     t0 = time.perf_counter()
-    for k in range(1, steps + 1):
-        f = s.step(f)
-        if k % every == 0:
-            g = f.float()
-            if dev_store:
-                g = g + D3Q19.W_EXACT64.to(g.device, g.dtype).reshape(19, 1, 1, 1)
-            _, ux, uy, uz = macroscopic3d(g.contiguous())
-            u = torch.stack([ux, uy, uz]).double()
-            ref = test[k // every][1:4].double()
-            errs.append(float(torch.linalg.vector_norm(u - ref) / torch.linalg.vector_norm(ref)))
-            kes.append(float((u * u).sum() / (ref * ref).sum()) - 1.0)
+    for k in range(every, steps + 1, every):
+        f, _ = s.run(f, every)
+        g = f.float()
+        if dev_store:
+            g = g + D3Q19.W_EXACT64.to(g.device, g.dtype).reshape(19, 1, 1, 1)
+        _, ux, uy, uz = macroscopic3d(g.contiguous())
+        u = torch.stack([ux, uy, uz]).double()
+        ref = test[k // every][1:4].double()
+        errs.append(float(torch.linalg.vector_norm(u - ref) / torch.linalg.vector_norm(ref)))
+        kes.append(float((u * u).sum() / (ref * ref).sum()) - 1.0)
     torch.cuda.synchronize()
     return {"err_final": errs[-1], "err_mean": sum(errs) / len(errs), "ke_err_final": kes[-1],
             "ke_err_mean_abs": sum(abs(v) for v in kes) / len(kes), "wall_s": time.perf_counter() - t0}
@@ -59,23 +59,32 @@ def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32,
 def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float = 0.51, steps: int = 300,
                            sample_every: int = 10, train_seed: int = 101, test_seed: int = 202, epochs: int = 30,
                            C_s: float = 0.17, les_dtype: torch.dtype = torch.float32,
-                           device: str | torch.device = "cuda:0", log=None) -> dict:
+                           device: str | torch.device = "cuda:0", log=None, k0: float = 4.0,
+                           u_rms: float = 0.03, max_samples: int = 1 << 25) -> dict:
     def say(msg):
         if log is not None:
             log(f"[les_pipeline {time.perf_counter() - t0:7.1f} s] {msg}")
 
     t0 = time.perf_counter()
-    train = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, train_seed, device, dns_scale=dns_scale)
+    train = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, train_seed, device, dns_scale=dns_scale,
+                                    k0=k0, u_rms=u_rms)
     x, t = zip(*(cl.sgs_samples(s) for s in train[1:]))
     del train
     feats, labels = torch.cat(x), torch.cat(t)
-    say(f"training DNS done: {feats.shape[0]} samples")
     del x, t
+    say(f"training DNS done: {feats.shape[0]} samples")
+    if feats.shape[0] > max_samples:  # a seeded random subset bounds the training time
+        keep = torch.randperm(feats.shape[0], generator=torch.Generator().manual_seed(train_seed))[:max_samples]
+        keep = keep.to(feats.device)
+        feats, labels = feats.index_select(0, keep), labels.index_select(0, keep)
+        say(f"kept {max_samples} of them")
     t_dns = time.perf_counter() - t0
-    model, meta = cl.train_eddy_viscosity_model_3d(feats, labels, cl.TrainConfig3D(epochs=epochs), device=device)
+    model, meta = cl.train_eddy_viscosity_model_3d(feats, labels, cl.TrainConfig3D(epochs=epochs), device=device,
+                                                   log=say)
     del feats, labels
     say(f"trained: val R2 {meta['final_val_r2']:.3f} in {meta['training_time_s']:.1f} s")
-    test = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, test_seed, device, dns_scale=dns_scale)
+    test = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, test_seed, device, dns_scale=dns_scale,
+                                   k0=k0, u_rms=u_rms)
     tau = cl.les_tau(tau_dns, dns_scale)
     say("test DNS done")
     runs = {}
@@ -83,6 +92,7 @@ def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float =
         runs[name] = _les(test, tau, steps, sample_every, dtype=les_dtype, device=device, **kw)
         say(f"LES {name}: {runs[name]}")
     return {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
+            "k0": k0, "u_rms": u_rms,
             "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
             "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
 
@@ -96,9 +106,12 @@ def main():
     p.add_argument("--every", type=int, default=10)
     p.add_argument("--epochs", type=int, default=30)
     p.add_argument("--les-dtype", choices=("float32", "float16"), default="float32")
+    p.add_argument("--k0", type=float, default=4.0, help="spectrum peak of the initial field (box wavenumbers)")
+    p.add_argument("--u-rms", type=float, default=0.03)
     a = p.parse_args()
     res = run_ai_les_pipeline_3d(a.n_les, a.dns_scale, a.tau_dns, a.steps, a.every, epochs=a.epochs,
-                                 les_dtype=getattr(torch, a.les_dtype), log=lambda m: print(m, flush=True))
+                                 les_dtype=getattr(torch, a.les_dtype), log=lambda m: print(m, flush=True),
+                                 k0=a.k0, u_rms=a.u_rms)
     print(json.dumps(res), flush=True)
 
 
