@@ -15,13 +15,9 @@ enum StoreKind { ST_F32 = 0, ST_F64 = 1, ST_F16 = 2, ST_BF16 = 3, ST_F16_DEV = 4
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// round to nearest even, NaN kept quiet
-__device__ __forceinline__ uint16_t f32_to_bf16(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
+// round to nearest even (the hardware conversion, v_cvt_pk_bf16_f32), NaN kept quiet
+__device__ __forceinline__ uint16_t f32_to_bf16(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
+
 
 template <int S>
 struct Store;
