@@ -198,29 +198,58 @@ struct MrtTab<D3Q27> {
 
 // f* = f - M^-1 S M (f - f_eq); S holds the per-moment rates, with the shear
 // entries given by omega (may vary per cell).  solver3d.py:107-160.
+// Relaxation group of MRT moment i (rows of MrtTab<L>::M; rates in the order the
+// reference's s vectors list them, solver.py:249-335, solver3d.py:64-90, d3q27.py:643-726):
+// conserved moments are never relaxed, the shear rows take omega.
+enum MrtGroup { MRT_CONSERVED, MRT_SE, MRT_SEPS, MRT_SQ, MRT_SPI, MRT_ONE, MRT_SHEAR };
+
+template <class L>
+constexpr int mrt_group(int i) {
+  if (L::Q == 9) {
+    constexpr int g[9] = {MRT_CONSERVED, MRT_SE, MRT_SEPS, MRT_CONSERVED, MRT_SQ, MRT_CONSERVED, MRT_SQ,
+                          MRT_SHEAR, MRT_SHEAR};
+    return g[i];
+  } else if (L::Q == 19) {
+    constexpr int g[19] = {MRT_CONSERVED, MRT_SE, MRT_SEPS, MRT_CONSERVED, MRT_SQ, MRT_CONSERVED, MRT_SQ,
+                           MRT_CONSERVED, MRT_SQ, MRT_SHEAR, MRT_SHEAR, MRT_SHEAR, MRT_SHEAR, MRT_SHEAR,
+                           MRT_SPI, MRT_SPI, MRT_ONE, MRT_ONE, MRT_ONE};
+    return g[i];
+  } else {
+    return i < 4 ? MRT_CONSERVED
+                 : (i == 4 ? MRT_SE : (i < 10 ? MRT_SHEAR : (i < 19 ? MRT_SQ : (i == 19 ? MRT_SEPS : MRT_SPI))));
+  }
+}
+
+// rate = (s_e, s_eps, s_q, s_pi) of the collision parameters
 template <class L, typename C>
-__device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C* s, C omega) {
+__device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C* rate, C omega) {
   constexpr int Q = L::Q;
   C m[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
+    m[i] = C(0);
+    if (mrt_group<L>(i) == MRT_CONSERVED) continue;  // relaxed at rate 0: never formed
     C acc = C(0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       constexpr double zero = 0.0;
       if (MrtTab<L>::M[i][j] != zero) acc += C(MrtTab<L>::M[i][j]) * (f[j] - feq[j]);
     }
-    m[i] = acc;
+    const int g = mrt_group<L>(i);
+    const C r = g == MRT_SE     ? rate[0]
+                : g == MRT_SEPS ? rate[1]
+                : g == MRT_SQ   ? rate[2]
+                : g == MRT_SPI  ? rate[3]
+                : g == MRT_ONE  ? C(1)
+                                : omega;
+    m[i] = acc * r;
   }
-  // rate per moment: s[i] < 0 marks "shear" (use omega)
-#pragma unroll
-  for (int i = 0; i < Q; ++i) m[i] *= (s[i] < C(0) ? omega : s[i]);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     C acc = C(0);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      if (MrtTab<L>::Mi[q][i] != 0.0) acc += C(MrtTab<L>::Mi[q][i]) * m[i];
+      if (mrt_group<L>(i) != MRT_CONSERVED && MrtTab<L>::Mi[q][i] != 0.0) acc += C(MrtTab<L>::Mi[q][i]) * m[i];
     }
     f[q] -= acc;
   }

@@ -65,7 +65,6 @@ struct KArgs {
   int tau_mode, eq_floor, bc_kind, bc_faces;
   C omega, tau, cs_delta_sq, tau_cap;
   C rate[8];
-  C s[27];
   C bc_u[3];
 };
 
@@ -304,7 +303,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
         omega = C(1) / C(a.tau_eff[cell]);
       }
       if constexpr (COLL == TLBM_COLL_MRT) {
-        mrt<L>(f, feq, a.s, omega);
+        mrt<L>(f, feq, a.rate, omega);
       } else if constexpr (COLL == TLBM_COLL_CM) {
         cascaded<L>(f, feq, mc, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_CUMULANT) {
@@ -863,22 +862,6 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
       if (coll->mlp == nullptr) return fail(TLBM_ERR_INVALID, "TLBM_TAU_MLP needs the packed closure (coll->mlp)");
       if (lattice != TLBM_D3Q19 || coll->kind != TLBM_COLL_BGK)
         return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
-    }
-    if (coll->kind == TLBM_COLL_MRT) {
-      // MRT rate vector; negative marks the shear moments (rate = omega)
-      const double se = coll->rate[0], seps = coll->rate[1], sq = coll->rate[2], spi = coll->rate[3];
-      double s[27];
-      for (int i = 0; i < 27; ++i) s[i] = 0.0;
-      if (lattice == TLBM_D2Q9) {
-        const double v[9] = {0, se, seps, 0, sq, 0, sq, -1, -1};
-        for (int i = 0; i < 9; ++i) s[i] = v[i];
-      } else if (lattice == TLBM_D3Q19) {
-        const double v[19] = {0, se, seps, 0, sq, 0, sq, 0, sq, -1, -1, -1, -1, -1, spi, spi, 1, 1, 1};
-        for (int i = 0; i < 19; ++i) s[i] = v[i];
-      } else {
-        for (int i = 4; i < 27; ++i) s[i] = i == 4 ? se : (i < 10 ? -1 : (i < 19 ? sq : (i == 19 ? seps : spi)));
-      }
-      for (int i = 0; i < lattice; ++i) a.s[i] = C(s[i]);
     }
   }
   a.bc_kind = TLBM_BC_NONE;
