@@ -288,7 +288,7 @@ def main():
     ms = head["ms_per_step"]
     value = head["mlups"]
     per = head["steps_per_launch"]
-    kernel = ("k_lbm2<D3Q19, fp32, BGK, mask + channel, 64x6 tile> (two steps per launch)" if per == 2 else
+    kernel = ("k_lbm2<D3Q19, fp32, BGK, mask + channel, 64x8 tile> (two steps per launch)" if per == 2 else
               "k_lbm<D3Q19, fp32, BGK, stream, mask + boundaries>")
     line = {
         "metric": METRIC,

@@ -917,14 +917,16 @@ int launch_lbm_kind(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32
 
 // ---- two steps per launch ----------------------------------------------------
 
-// Tile of the two-step kernel: 64 x 6 output cells (9 waves, 1.38x halo reads)
-// where three planes fit the 160 KB of LDS, else 32 x 8 (6 waves; D3Q27 with
-// 4-byte storage).
+// Tile of the two-step kernel: 64 x 8 output cells (11 waves, 1.29x halo reads;
+// D3Q19 fp32: 150 KB for three planes) where three planes fit the 160 KB of LDS
+// next to the mask and f_eq tables, else 32 x 8 (6 waves; D3Q27 with 4-byte
+// storage).  Measured on the 256^3 channel: 64 x 6 46.0, 64 x 7 50.7, 64 x 8
+// 57.2 GLUPS (more waves per CU and less recomputed halo).
 template <class L, int S>
 struct Tile2 {
-  static constexpr bool wide = 3 * L::Q * 66 * 8 * sizeof(typename Store<S>::T) <= 160 * 1024;
+  static constexpr bool wide = 3 * L::Q * 66 * 10 * sizeof(typename Store<S>::T) <= 158 * 1024;
   static constexpr int TX = wide ? 64 : 32;
-  static constexpr int TY = wide ? 6 : 8;
+  static constexpr int TY = 8;
 };
 
 inline int tile2_width(int lattice, int store) {

@@ -241,10 +241,10 @@ class FusedSolver3D:
     ``temporal_blocking`` is on and the library builds the case: the same
     results as single steps, one HBM round trip of the populations per two
     steps.  ``step2()`` is one such pair.  ``None`` (default) turns it on for
-    D3Q19 BGK with fp32 storage, the form it speeds up on MI355X (256³
-    channel: 45.9 vs 38.9 GLUPS); with 16-bit storage, the moment-space
-    collisions and D3Q27 the recomputed tile halo and the longer per-wave
-    instruction stream make it slower than one step per launch.
+    D3Q19 with fp32 storage, where it pays on MI355X (256³ channel + sphere,
+    GLUPS one / two steps per launch: BGK 39.0 / 57.2, MRT 39.5 / 46.0, CM
+    39.4 / 43.3, cumulant 39.2 / 49.4); with 16-bit storage and for D3Q27 the
+    recomputed tile halo costs more instruction issue than the saved traffic.
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
@@ -317,7 +317,7 @@ class FusedSolver3D:
         self._buf_b: torch.Tensor | None = None
         self._report_step = 0
         if temporal_blocking is None:
-            temporal_blocking = lat == "D3Q19" and self._store == nat.STORE_F32 and col == "BGK"
+            temporal_blocking = lat == "D3Q19" and self._store == nat.STORE_F32
         self._two_step = bool(temporal_blocking) and lat != "D2Q9" and bool(
             nat.lib().tlbm_step2_supported(self._Q, self._store, self._coll, self._bc, self.nx))
 

@@ -7,7 +7,7 @@ different contexts and the compiler may fuse multiply-adds differently, which
 moves a value by an ulp of fp32 (or, rounded into 16-bit storage, by one ulp of
 the stored type).  The tests require that bar everywhere and identical values
 almost everywhere (measured bit-identical here).  Grids are ragged against the
-64 x 6 tiles (32 x 8 for D3Q27 fp32) and the z
+64 x 8 tiles (32 x 8 for D3Q27 fp32) and the z
 blocking, and one is smaller than a tile so the halo wraps more than once.  The
 integer canary pins the LDS addressing (including the z slots and bounce-back)
 to the oracle's pull.
@@ -217,9 +217,9 @@ def test_two_fused_bgk_steps_match_oracle():
 
 
 def test_default_policy():
-    """Two steps per launch by default only for D3Q19 BGK with fp32 storage."""
+    """Two steps per launch by default only for D3Q19 with fp32 storage."""
     assert tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32).two_step
-    assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, collision="MRT").two_step
+    assert tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, collision="MRT").two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float16, deviation=True).two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32, lattice="D3Q27").two_step
     assert not tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float64).two_step
