@@ -22,6 +22,21 @@ from .fused import (
     is_available,
 )
 from .cavities import BatchedCavities
+from .checkpoint import (
+    CheckpointError,
+    SolverCheckpoint,
+    absolute_populations,
+    eager_load_state_dict,
+    eager_state_dict,
+    fused_load_state_dict,
+    fused_state_dict,
+    load_checkpoint,
+    load_solver_checkpoint,
+    save_checkpoint,
+    save_solver_checkpoint,
+    triton_fused_load_state_dict,
+    triton_fused_state_dict,
+)
 from .closure import EddyViscosityMLP3D, TrainConfig3D, train_eddy_viscosity_model_3d
 from .executor import LBMStepExecutor
 from .ops import *  # noqa: F401,F403
@@ -44,5 +59,8 @@ __all__ = list(_ops_all) + [
     "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
     "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",
     "TritonFusedSolver3D", "LBMStepExecutor", "BatchedCavities", "EddyViscosityMLP3D", "TrainConfig3D",
-    "train_eddy_viscosity_model_3d",
+    "train_eddy_viscosity_model_3d", "CheckpointError", "SolverCheckpoint", "absolute_populations",
+    "eager_load_state_dict", "eager_state_dict", "fused_load_state_dict", "fused_state_dict", "load_checkpoint",
+    "load_solver_checkpoint", "save_checkpoint", "save_solver_checkpoint", "triton_fused_load_state_dict",
+    "triton_fused_state_dict",
 ]

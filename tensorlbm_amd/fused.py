@@ -310,6 +310,7 @@ class FusedSolver3D:
             raise ValueError(f"unknown boundary {boundary!r}; valid: channel_x, lid_y, far_field")
         faces = nat.far_field_faces(bc_config) if boundary == "far_field" else 0
         self._bc = nat.make_boundary(kinds[boundary], u_bc, faces) if boundary is not None else None
+        self.boundary = boundary
         if layout not in ("padded", "contiguous"):
             raise ValueError(f"layout must be 'padded' or 'contiguous', got {layout!r}")
         self.layout = layout
@@ -399,6 +400,21 @@ class FusedSolver3D:
                 break
         self._report_step = step
         return f, diags
+
+    def state_dict(self, f: torch.Tensor, *, step: int | None = None, extra=None) -> dict:
+        """Checkpointable state of populations ``f`` (``triton_fused_state_dict``, checkpoint.py:568-586):
+        lattice, Q, grid, τ, precision tier, storage dtype and offset convention; ``step`` defaults to the
+        ``run()`` counter."""
+        from .checkpoint import fused_state_dict
+
+        return fused_state_dict(self, f, step=step, extra=extra)
+
+    def load_state_dict(self, state, *, device=None) -> torch.Tensor:
+        """Populations of ``state`` on this solver's device after a fail-closed identity check
+        (``triton_fused_load_state_dict``, checkpoint.py:589-617); resets the ``run()`` counter."""
+        from .checkpoint import fused_load_state_dict
+
+        return fused_load_state_dict(self, state, device=device)
 
     def grid_size_cells(self) -> int:
         return self.nz * self.ny * self.nx

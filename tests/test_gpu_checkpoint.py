@@ -1,0 +1,124 @@
+"""Resume from a checkpoint on the GPU: a restored run continues bit-identically.
+
+The reference's round-trip test (tests/test_checkpoint.py:437-471) runs three
+fused steps, checkpoints, restores into a fresh solver and requires the next
+three steps to equal an uninterrupted six.  Here the same for the fp32
+channel + sphere case (pairs of steps per launch, split on a pair boundary so
+both runs launch the same kernels), for fp16 deviation storage (offset
+convention recorded and enforced), for the tensor + JSON pair written from the
+solver's padded views, and for the eager executor.
+"""
+
+from __future__ import annotations
+
+import pytest
+import torch
+
+import tensorlbm_amd as tl
+from oracle import ref
+from tests import _fields as F
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _channel(dtype=torch.float32, deviation=False, **kw):
+    nz, ny, nx = 12, 18, 70
+    solid = torch.from_numpy(ref.sphere_mask(nx, ny, nz, 20.0, 9.0, 6.0, 4.5)).to(torch.int8)
+    return tl.FusedSolver3D(nz, ny, nx, 0.56, DEV, dtype, solid=solid, boundary="channel_x",
+                            u_bc=(0.05, 0.0, 0.0), deviation=deviation, **kw)
+
+
+def _initial(shape, dtype=torch.float32, deviation=False, seed=5):
+    f = F.random_state(19, shape, seed=seed, noise=2e-3, u_amp=0.02)
+    if deviation:
+        f = f - F.LATS[19].w.reshape(-1, 1, 1, 1)
+    return torch.from_numpy(f).to(DEV, dtype=dtype)
+
+
+def test_fp32_channel_resume_is_bitwise(tmp_path):
+    whole = _channel()
+    assert whole.two_step
+    f0 = _initial((12, 18, 70))
+    want, _ = whole.run(f0, 6)
+
+    first = _channel()
+    f, _ = first.run(f0, 2)
+    path = tl.save_solver_checkpoint(tmp_path / "ch.ckpt", first.state_dict(f), metadata={"case": "sphere"})
+    c = tl.load_solver_checkpoint(path, expected_lattice="D3Q19", expected_grid=(12, 18, 70), expected_q=19)
+    assert c.step == 2 and c.state["storage_offset"] == "none" and c.state["boundary"] == "channel_x"
+    assert torch.equal(c.state["obstacle_mask"], first.solid.cpu())
+
+    second = _channel()
+    g = second.load_state_dict(c.state)
+    assert second._report_step == 2 and g.device.type == "cuda"
+    got, _ = second.run(g, 4)
+    torch.cuda.synchronize()
+    assert second._report_step == 6
+    assert torch.equal(got, want)
+
+
+def test_fp16_deviation_resume_is_bitwise_and_offset_enforced(tmp_path):
+    shape = (10, 12, 16)
+    mk = lambda **kw: tl.FusedSolver3D(*shape, 0.6, DEV, torch.float16, deviation=True, **kw)  # noqa: E731
+    f0 = _initial(shape, torch.float16, deviation=True)
+    want, _ = mk().run(f0, 6)
+
+    first = mk()
+    f, _ = first.run(f0, 3)
+    path = tl.save_solver_checkpoint(tmp_path / "dev.ckpt", first.state_dict(f))
+    c = tl.load_solver_checkpoint(path)
+    assert c.state["storage_offset"] == "f-w" and c.state["storage_dtype"] == "torch.float16"
+    with pytest.raises(tl.CheckpointError, match="storage offset"):
+        tl.FusedSolver3D(*shape, 0.6, DEV, torch.float16).load_state_dict(c.state)
+
+    second = mk()
+    got, _ = second.run(second.load_state_dict(c.state), 3)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+
+    # periodic BGK conserves mass: the absolute populations after three steps hold the initial mass
+    # (up to fp16 rounding of the stored deviations)
+    w = F.LATS[19].w.reshape(-1, 1, 1, 1)
+    mass0 = (f0.double().cpu() + torch.from_numpy(w)).sum().item()
+    mass3 = tl.absolute_populations(c.state).double().sum().item()
+    assert abs(mass3 - mass0) < 1e-3 * f0[0].numel()
+
+
+def test_pair_files_from_padded_views(tmp_path):
+    s = _channel(temporal_blocking=False)
+    f, _ = s.run(_initial((12, 18, 70)), 3)
+    assert not f.is_contiguous()
+    tl.save_checkpoint(f, 3, tmp_path, extra={"solver": "FusedSolver3D"})
+    g, step, meta = tl.load_checkpoint(tmp_path, DEV, expected_shape=(19, 12, 18, 70), expected_lattice_directions=19)
+    assert step == 3 and meta["solver"] == "FusedSolver3D" and g.device.type == "cuda"
+    assert torch.equal(g, f)
+    # a dense tensor is a valid input to the padded solver: same next step
+    a = s.step(f).clone()
+    b = _channel(temporal_blocking=False).step(g)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_eager_executor_resume_is_bitwise(tmp_path):
+    shape = (6, 8, 10)
+    mk = lambda: tl.LBMStepExecutor("D3Q19", collide_fn="bgk", device=DEV, nx=10, ny=8, nz=6, tau=0.7,  # noqa: E731
+                                    dtype=torch.float32)
+    f0 = _initial(shape)
+    ex = mk()
+    want = f0
+    for _ in range(4):
+        want, _ = ex.step(want)
+
+    ex = mk()
+    f = f0
+    for _ in range(2):
+        f, _ = ex.step(f)
+    path = tl.save_solver_checkpoint(tmp_path / "eager.ckpt", tl.eager_state_dict(ex, f, step=2))
+    ex2 = mk()
+    g = tl.eager_load_state_dict(ex2, tl.load_solver_checkpoint(path).state)
+    for _ in range(2):
+        g, _ = ex2.step(g)
+    torch.cuda.synchronize()
+    assert ex2._report_step == 2 and torch.equal(g, want)
