@@ -38,6 +38,7 @@ from .checkpoint import (
     triton_fused_state_dict,
 )
 from .closure import EddyViscosityMLP3D, TrainConfig3D, train_eddy_viscosity_model_3d
+from .diagnostics import frontal_area, obstacle_drag_coefficient, voxel_wetted_area
 from .executor import LBMStepExecutor
 from .ops import *  # noqa: F401,F403
 from .ops import __all__ as _ops_all
@@ -62,5 +63,5 @@ __all__ = list(_ops_all) + [
     "train_eddy_viscosity_model_3d", "CheckpointError", "SolverCheckpoint", "absolute_populations",
     "eager_load_state_dict", "eager_state_dict", "fused_load_state_dict", "fused_state_dict", "load_checkpoint",
     "load_solver_checkpoint", "save_checkpoint", "save_solver_checkpoint", "triton_fused_load_state_dict",
-    "triton_fused_state_dict",
+    "triton_fused_state_dict", "frontal_area", "obstacle_drag_coefficient", "voxel_wetted_area",
 ]
