@@ -27,6 +27,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "../../include/tlbm_hip.h"
@@ -938,11 +939,38 @@ constexpr bool step2_store() {
   return L::Q != 9 && (S == ST_F32 || S == ST_F16_DEV || S == ST_BF16_DEV);
 }
 
-// z planes per workgroup: 32, fewer while the grid is below four workgroups per CU
-inline int step2_zc(int cols, int nz) {
-  int zc = 32;
-  while (zc > 8 && int64_t(cols) * ((nz + zc - 1) / zc) < 1024) zc /= 2;
-  return zc;
+// Compute units of the current device (cached per device).
+inline int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cache[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+// z planes per workgroup.  A two-step workgroup holds most of a CU's LDS, so a CU
+// runs one at a time and a launch of W = cols * ceil(nz / zc) workgroups takes
+// ceil(W / CUs) rounds of about zc + 2 plane times (zc planes, the two-plane fill
+// of the march at half cost, the prologue).  Pick the zc with the least total:
+// on MI355X at 256^3 that is 128 (one round of 256 workgroups; 60.7 GLUPS on the
+// channel + sphere against 59.1 / 57.2 / 54.8 for zc = 64 / 32 / 16, whose
+// costs this model ranks and spaces the same way).
+inline int step2_zc(int cols, int nz, int cus) {
+  int best = nz;
+  int64_t best_cost = INT64_MAX;
+  for (int zc = 1; zc <= nz; ++zc) {
+    const int64_t w = int64_t(cols) * ((nz + zc - 1) / zc);
+    const int64_t cost = (w + cus - 1) / cus * (zc + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  return best;
 }
 
 // why a two-step launch cannot serve these arguments (nullptr: it can)
@@ -963,7 +991,7 @@ template <class L, int S, int COLL, int FEAT, int BCK>
 int launch_lbm2(const KArgs<typename Store<S>::C>& a, hipStream_t s) {
   constexpr int TX = Tile2<L, S>::TX, TY = Tile2<L, S>::TY;
   const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY;
-  const int zc = step2_zc(gx * gy, a.nz);
+  const int zc = step2_zc(gx * gy, a.nz, device_cus());
   const uint32_t tiles = uint32_t(gx) * uint32_t(gy) * uint32_t((a.nz + zc - 1) / zc);
   hipLaunchKernelGGL((k_lbm2<L, S, COLL, FEAT, BCK, TX, TY>), dim3(tiles), dim3(tb_threads<TX, TY>()), 0, s, a, zc);
   return check_launch("k_lbm2");
