@@ -420,6 +420,28 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t w, uint32_t n) {
 // stays in flight across the barrier.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Rows of the two-step kernel's LDS ring.  Step two on plane z pulls population q
+// from plane z - c_z[q], so the c_z = -1 populations of a plane are read for one
+// plane of the march, c_z = 0 for two and c_z = +1 for three; one more slot per
+// set lets the next plane be written while this one is still read.
+template <class L>
+struct ZRing {
+  static constexpr int count(int v) {
+    int n = 0;
+    for (int q = 0; q < L::Q; ++q) n += L::cz[q] == v ? 1 : 0;
+    return n;
+  }
+  // position of q among the populations with the same c_z
+  static constexpr int index(int q) {
+    int n = 0;
+    for (int r = 0; r < q; ++r) n += L::cz[r] == L::cz[q] ? 1 : 0;
+    return n;
+  }
+  static constexpr int NA = count(-1), NB = count(0), NC = count(1);
+  static constexpr int OFF_B = 2 * NA, OFF_C = OFF_B + 3 * NB;
+  static constexpr int ROWS = OFF_C + 4 * NC;  // 57 for D3Q19 (= 3 Q), 81 for D3Q27
+};
+
 // Two fused steps per launch (temporal blocking).  A workgroup owns a TX x TY
 // column of output cells and marches z over `zc` planes.  Step-one states of a
 // (TX+2) x (TY+2) tile (one cell per lane, read from HBM) are kept in LDS for
@@ -436,9 +458,15 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
   static_assert((FEAT & (FEAT_MLP | FEAT_FORCE)) == 0, "the closure and the force sample run in the one-step kernel");
-  __shared__ T s[3 * Q * EC];  // step-one states of three planes, [slot][q][tile cell]
-  // solid flags of the same tile cells: step two takes them from here rather than
-  // waiting on a mask load of its own each plane
+  // step-one states, one row of EC tile cells per (set, slot, population): the
+  // c_z = -1 / 0 / +1 populations of a plane are pulled while it is the newest /
+  // middle / oldest of three, and each set has one slot more than it has planes
+  // in use, so step one of the next plane can be written while slower waves
+  // still pull for this one: one barrier per plane
+  using R = ZRing<L>;
+  __shared__ T s[R::ROWS * EC];
+  // solid flags of the same tile cells (three planes): step two takes them from
+  // here rather than waiting on a mask load of its own each plane
   __shared__ uint8_t s_solid[kAll ? 3 * EC : 1];
   // f_eq(1, bc_u), built once per workgroup with the code the one-step kernel runs
   __shared__ C s_eq[kAll ? Q : 1];
@@ -497,7 +525,11 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     }
     gather<L, S>(pre, fin, a, gxn, gy, pz);
   };
-  auto step_one = [&](int p, int slot) {
+  // first row of the set holding population q, for the plane written j-th in the march
+  auto set_row = [](int cz, int j) {
+    return cz < 0 ? (j & 1) * R::NA : cz == 0 ? R::OFF_B + (j % 3) * R::NB : R::OFF_C + (j & 3) * R::NC;
+  };
+  auto step_one = [&](int p, int j) {
     if (!h1) return;
     const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
     const uint32_t cell = (uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx);
@@ -508,27 +540,37 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     const bool solid = pre_mask != 0;
     update_cell<L, S, COLL, true, FEAT, BCN_IN_F, BCK>(f, a, gx, gy, pz, cell, solid, nb_edge, unused_force,
                                                        unused_force, unused_force, unused, eq_unit);
+    const int row_a = set_row(-1, j), row_b = set_row(0, j), row_c = set_row(1, j);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s[(slot * Q + q) * EC + t] = ST::template put<L>(f[q], q);
-    if constexpr (kAll) s_solid[slot * EC + t] = solid ? 1 : 0;
+    for (int q = 0; q < Q; ++q) {
+      const int row = (L::cz[q] < 0 ? row_a : L::cz[q] == 0 ? row_b : row_c) + R::index(q);
+      s[row * EC + t] = ST::template put<L>(f[q], q);
+    }
+    if constexpr (kAll) s_solid[(j % 3) * EC + t] = solid ? 1 : 0;
   };
 
-  // iteration k computes step one on plane z0 + k + 1 (slot (k + 2) % 3) and, from
-  // k = 0 on, step two on plane z = z0 + k; k = -2 and -1 fill the first two planes
+  // iteration k computes step one on plane z0 + k + 1 (the (k + 2)-th plane of the
+  // march) and, from k = 0 on, step two on plane z = z0 + k; k = -2 and -1 fill the
+  // first two planes.  Between two barriers a wave may run step two of iteration k
+  // while another runs step one of k + 1: the rows that step one writes (set slots
+  // (k + 3) mod 2 / 3 / 4) are not among those step two of k reads (k mod 2,
+  // (k + 1) mod 3, k mod 4), and no wave passes the next barrier before all are done.
   prefetch(z0 - 1);
   for (int k = -2; z0 + k < zend; ++k) {
     const int z = z0 + k;
     const int p1 = z + 1;
-    step_one(p1, (k + 2) % 3);
+    step_one(p1, k + 2);
     if (p1 < zend) prefetch(p1 + 1);  // the next iteration's step-one plane
     lds_barrier();
     if (k >= 0 && st) {
       // step-one value of population q pulled into cell (x, y) of plane z (same
-      // plane only: the boundary overrides read along x)
+      // plane only: the boundary overrides read along x), written as plane
+      // k + 1 - c_z[q] of the march
+      const int row_a = set_row(-1, k + 2), row_b = set_row(0, k + 1), row_c = set_row(1, k);
       auto lds_at = [&](int q, int x, int y, int) -> C {
-        const int slot = (k + 1 - L::cz[q] + 3) % 3;
+        const int row = (L::cz[q] < 0 ? row_a : L::cz[q] == 0 ? row_b : row_c) + R::index(q);
         const int sx = x - L::cx[q] - (x0 - 1), sy = y - L::cy[q] - (y0 - 1);
-        return ST::template get<L>(s[(slot * Q + q) * EC + sy * EX + sx], q);
+        return ST::template get<L>(s[row * EC + sy * EX + sx], q);
       };
       C f[Q];
 #pragma unroll
@@ -543,7 +585,6 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
 #pragma unroll
       for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
     }
-    lds_barrier();
   }
 }
 
