@@ -69,10 +69,18 @@ struct KArgs {
   C bc_u[3];
 };
 
+// Population buffers are device (global) memory.  Saying so in the pointer type
+// makes the compiler emit global_load / global_store; through a generic pointer
+// it emits FLAT operations, which count in both the vector-memory and the LDS
+// counters, so every wait for LDS (the workgroup barrier's) would also wait for
+// the outstanding stores to reach memory.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+
 // value at a uniform base plus a per-lane 32-bit element offset
 template <typename T>
 __device__ __forceinline__ T ld_off(const T* base, uint32_t off) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off * uint32_t(sizeof(T)));
+  return *(global_ptr<const T>)(reinterpret_cast<const char*>(base) + off * uint32_t(sizeof(T)));
 }
 
 // A pointer the caller knows to be uniform over the wave, moved to scalar registers
@@ -89,8 +97,7 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 // non-temporal store to a wave-uniform base plus a per-lane 32-bit element offset
 template <typename T>
 __device__ __forceinline__ void st_nt(T* base, uint32_t off, T v) {
-  __builtin_nontemporal_store(
-      v, reinterpret_cast<T*>(reinterpret_cast<char*>(uniform_ptr(base)) + off * uint32_t(sizeof(T))));
+  __builtin_nontemporal_store(v, (global_ptr<T>)(reinterpret_cast<char*>(uniform_ptr(base)) + off * uint32_t(sizeof(T))));
 }
 
 __device__ __forceinline__ int wrap_dn(int v, int n) { return v == 0 ? n - 1 : v - 1; }
@@ -415,6 +422,17 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t w, uint32_t n) {
   return c < r ? c * (q + 1) + slot : r * (q + 1) + (c - r) * q + slot;
 }
 
+// Vector-memory counter of gfx9-family parts: it counts loads and stores, in
+// issue order.  Step one of the next plane needs only the loads prefetched
+// before this plane's step-two stores, i.e. a wait until at most Q operations
+// remain.  The compiler takes the strictest wait over the paths that reach the
+// loop head, and the paths that issue no stores (the march's entry, the waves
+// and fill planes without step-two work) would force vmcnt(0), i.e. waiting for
+// the stores to reach memory as well.  Those paths wait for their loads here
+// instead, so the merged wait at the loop head becomes vmcnt(Q).
+// Encoding: vmcnt [3:0] and [15:14] = 0, expcnt [6:4] = 7, lgkmcnt [11:8] = 15.
+__device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // LDS writes complete, then the workgroup barrier.  Unlike __syncthreads() this
 // does not wait for outstanding global loads, so the next plane's prefetch
 // stays in flight across the barrier.
@@ -492,11 +510,15 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
               (bck == TLBM_BC_FAR_FIELD && gx == nx - 1 && !(a.bc_faces & TLBM_FACE_XP_PERIODIC));
   }
   const int gxn = nb_edge ? (gx == 0 ? 1 : nx - 2) : gx;
-  // output cell of this lane
+  // output cell of this lane.  The output lanes fill whole waves; lanes of a
+  // ragged tile whose cell lies past the grid take the tile's last in-grid cell
+  // instead, compute that cell from the same inputs with the same code and store
+  // the same bits as its own lane, so the stores need no lane mask (see the loop)
+  static_assert(TX * TY % 64 == 0, "output lanes fill whole waves");
+  const bool out_wave = __builtin_amdgcn_readfirstlane(t) < TX * TY;
   const bool h2 = t < TX * TY;
-  const int ox = h2 ? t % TX : 0, oy = h2 ? t / TX : 0;
-  const int ux_ = x0 + ox, uy_ = y0 + oy;
-  const bool st = h2 && ux_ < nx && uy_ < ny;
+  const int ux_ = min(x0 + (h2 ? t % TX : 0), nx - 1), uy_ = min(y0 + (h2 ? t / TX : 0), ny - 1);
+  const int ox = ux_ - x0, oy = uy_ - y0;
   double unused_force = 0.0;  // no force sample here: tlbm_step2 takes no force buffer
 
   if constexpr (kAll) {
@@ -549,20 +571,32 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     if constexpr (kAll) s_solid[(j % 3) * EC + t] = solid ? 1 : 0;
   };
 
-  // iteration k computes step one on plane z0 + k + 1 (the (k + 2)-th plane of the
-  // march) and, from k = 0 on, step two on plane z = z0 + k; k = -2 and -1 fill the
-  // first two planes.  Between two barriers a wave may run step two of iteration k
-  // while another runs step one of k + 1: the rows that step one writes (set slots
-  // (k + 3) mod 2 / 3 / 4) are not among those step two of k reads (k mod 2,
-  // (k + 1) mod 3, k mod 4), and no wave passes the next barrier before all are done.
+  // The march: iteration k computes step one on plane z0 + k + 1 (the (k + 2)-th
+  // plane written) and, from k = 0 on, step two on plane z = z0 + k.  Between two
+  // barriers a wave may run step two of iteration k while another runs step one
+  // of k + 1: the rows that step one writes (set slots (k + 3) mod 2 / 3 / 4) are
+  // not among those step two of k reads (k mod 2, (k + 1) mod 3, k mod 4), and no
+  // wave passes the next barrier before all are done.
+  //
+  // Waits for the prefetched loads: every path into the main loop's head either
+  // issued its step-two stores after the loads or waited for the loads itself
+  // (the fill planes, the waves without output cells), so the wait at the head
+  // is vmcnt(Q) and the previous plane's stores stay in flight.
   prefetch(z0 - 1);
-  for (int k = -2; z0 + k < zend; ++k) {
+  for (int k = -2; k < 0; ++k) {  // the two fill planes: step one only
+    const int p1 = z0 + k + 1;
+    step_one(p1, k + 2);
+    prefetch(p1 + 1);  // p1 + 1 <= z0 + 1 <= zend
+    lds_barrier();
+  }
+  wait_vm_loads();
+  for (int k = 0; z0 + k < zend; ++k) {
     const int z = z0 + k;
     const int p1 = z + 1;
     step_one(p1, k + 2);
     if (p1 < zend) prefetch(p1 + 1);  // the next iteration's step-one plane
     lds_barrier();
-    if (k >= 0 && st) {
+    if (out_wave) {
       // step-one value of population q pulled into cell (x, y) of plane z (same
       // plane only: the boundary overrides read along x), written as plane
       // k + 1 - c_z[q] of the march
@@ -584,6 +618,8 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
       const uint32_t off = uint32_t(uy_) * uint32_t(a.osy) + uint32_t(ux_);
 #pragma unroll
       for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
+    } else {
+      wait_vm_loads();
     }
   }
 }
