@@ -216,6 +216,57 @@ def test_two_fused_bgk_steps_match_oracle():
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-6)
 
 
+def _headline_state(n=256, seed=21):
+    """256³ channel + sphere (radius 24 at x = 64, y = z = 128): equilibrium of u = (0.05, 0, 0) with a seeded
+    1e-3 velocity perturbation, built on the GPU."""
+    dev = "cuda:0"
+    a = torch.arange(n, dtype=torch.float32, device=dev)
+    z, y, x = torch.meshgrid(a, a, a, indexing="ij")
+    solid = ((x - 64.0) ** 2 + (y - n / 2) ** 2 + (z - n / 2) ** 2) <= 24.0**2
+    del x, y, z
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    pert = ((torch.rand((3, n, n, n), generator=g) * 2 - 1) * 1e-3).to(dev)
+    one = torch.ones((n, n, n), device=dev)
+    f = tl.equilibrium3d(one, 0.05 + pert[0], pert[1].contiguous(), pert[2].contiguous())
+    return solid.to(torch.int8), f
+
+
+def test_headline_workload_two_steps_match_single_steps():
+    """The benchmarked launch itself (256³, 256 workgroups of 128 planes): one two-step launch = two steps."""
+    n = 256
+    solid, f0 = _headline_state(n)
+    kw = dict(solid=solid, boundary="channel_x", u_bc=(0.05, 0.0, 0.0))
+    one = tl.FusedSolver3D(n, n, n, 0.572, "cuda:0", torch.float32, temporal_blocking=False, **kw)
+    two = tl.FusedSolver3D(n, n, n, 0.572, "cuda:0", torch.float32, temporal_blocking=True, **kw)
+    assert two.two_step
+    a, b, _, _ = _pair(one, two, f0, pairs=2)
+    _assert_equal(a, b)
+
+
+def test_full_size_periodic_mass_conserved():
+    """256³ periodic BGK, 10 two-step launches: the FP64 sum of the populations moves by fp32 rounding only."""
+    n = 256
+    _, f0 = _headline_state(n, seed=22)
+    two = tl.FusedSolver3D(n, n, n, 0.6, "cuda:0", torch.float32)
+    assert two.two_step
+    m0 = tl.moment_sums(f0)[0].item()
+    f = f0
+    for _ in range(10):
+        f = two.step2(f)
+    m1 = tl.moment_sums(f)[0].item()
+    assert abs(m1 - m0) <= 1e-6 * m0, (m0, m1)
+
+
+def test_truncated_last_z_chunk():
+    """200 planes in chunks of 7 (the launch-shape policy for 8 columns on 256 CUs): the last chunk has 4."""
+    shape = (200, 64, 64)
+    solid = torch.from_numpy(ref.sphere_mask(64, 64, 200, 20.0, 32.0, 100.0, 9.5)).to(torch.int8)
+    one, two = _solvers(19, shape, 0.58, solid=solid, boundary="channel_x", u_bc=(0.04, 0.0, 0.0))
+    assert two.two_step
+    a, b, _, _ = _pair(one, two, _initial(19, shape, "fp32", seed=23, u_amp=0.02), pairs=2)
+    _assert_equal(a, b)
+
+
 def test_default_policy():
     """Two steps per launch by default only for D3Q19 with fp32 storage."""
     assert tl.FusedSolver3D(8, 8, 64, 0.6, "cuda:0", torch.float32).two_step
