@@ -269,7 +269,12 @@ def main():
         closure.set_feature_stats(torch.zeros(7), torch.full((7,), 1e-3))
         forms["bgk_fp16dev_mlp_channel_sphere"] = form(torch.float16, deviation=True, mlp=True)
         forms["bgk_fp32_mlp_channel_sphere"] = form(torch.float32, mlp=True)
-        forms["fp16dev_mlp_speedup_vs_fp32"] = round(forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"] / head["mlups"], 3)
+        # the closure form against the headline (fp32, two steps per launch) and against the plain
+        # one-step fp32 kernel
+        mlp_rate = forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"]
+        forms["fp16dev_mlp_speedup_vs_fp32"] = round(mlp_rate / head["mlups"], 3)
+        forms["fp16dev_mlp_speedup_vs_fp32_one_step"] = round(
+            mlp_rate / forms["bgk_fp32_channel_sphere_one_step_per_launch"]["mlups"], 3)
         skip = {x.strip() for x in args.skip.split(",") if x.strip()}
         if "cavities" not in skip:
             forms["cavities_64"] = cavities_form(torch, dev)
