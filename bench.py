@@ -169,7 +169,8 @@ def dhit_forms(torch, dev, n, steps=100):
 
 def cavities_form(torch, dev, steps=100):
     """64 lid-driven cavities (edges U{32..160}, Re log-uniform [100, 1000]): one autotuned launch per step
-    replayed from a HIP graph, against 64 single-cavity fused launches per step."""
+    (or per two steps, when the autotuner picks the two-step launch) replayed from a HIP graph, against 64
+    single-cavity fused launches per step."""
     import tensorlbm_amd as tl
     from tensorlbm_amd.cavities import BatchedCavities, cavity_batch_config, wall_mask
 
@@ -177,6 +178,7 @@ def cavities_form(torch, dev, steps=100):
     b = BatchedCavities(edges, taus, u_lid=0.1, device=dev, dtype=torch.float32)
     b.fill_equilibrium()
     tune = b.autotune()
+    two_step = b.two_step
     ms_graph = b.time_steps(steps, graph=True, graph_steps=20) * 1e3
     ms_eager = b.time_steps(steps // 2, graph=False) * 1e3
     cells = b.num_cells
@@ -198,7 +200,8 @@ def cavities_form(torch, dev, steps=100):
     ms_seq, _, _ = time_steps(seq_step, None, steps // 4, 3, torch)
     del solvers
     torch.cuda.empty_cache()
-    return {"cells": cells, "edges_min_max": [min(edges), max(edges)], "grid": tune and min(tune, key=tune.get),
+    return {"cells": cells, "edges_min_max": [min(edges), max(edges)], "shape": tune and min(tune, key=tune.get),
+            "steps_per_launch": 2 if two_step else 1,
             "autotune_ms": {str(k): round(v * 1e3, 4) for k, v in tune.items()},
             "ms_per_step_graph": round(ms_graph, 4), "mlups_graph": round(cells / (ms_graph * 1e-3) / 1e6, 1),
             "ms_per_step_eager": round(ms_eager, 4), "ms_per_step_64_sequential": round(ms_seq, 4),

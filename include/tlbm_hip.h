@@ -272,6 +272,18 @@ int tlbm_cavities_step(int store, const void* f_in, void* f_out, int64_t capacit
 typedef struct tlbm_graph tlbm_graph;
 int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const tlbm_cavity* cavs, int n,
                         double u_lid, int grid, int steps, tlbm_graph** graph);
+/* two steps of every cavity in one launch, f_in -> f_out (store F32, F16_DEV or BF16_DEV):
+ * the two-step march of tlbm_step2 over 64 x 8 column tiles of every cavity in one grid, with
+ * the walls decided from the coordinates -- per cavity the same result as two tlbm_step
+ * launches with the wall mask and TLBM_BC_LID_Y, up to rounding of the stored type.
+ * zc = 0: the z chunk per workgroup from the launch-shape model; zc > 0: that many planes.
+ * Two iterations of the reference's per-cavity step loop (cases/cavity.py) for all cavities. */
+int tlbm_cavities_step2(int store, const void* f_in, void* f_out, int64_t capacity, const tlbm_cavity* cavs,
+                        int n, double u_lid, int zc, tlbm_stream_t stream);
+/* Capture `launches` (even) two-step launches a -> b -> a ... (2 x launches steps) into a HIP
+ * graph; after a replay the state is back in f_a. */
+int tlbm_cavities_graph2(int store, void* f_a, void* f_b, int64_t capacity, const tlbm_cavity* cavs, int n,
+                         double u_lid, int zc, int launches, tlbm_graph** graph);
 /* enqueue one replay on `stream`; on the null stream the replay runs on the graph's own stream,
  * fenced by events so that it follows earlier and precedes later null-stream work */
 int tlbm_graph_launch(tlbm_graph* graph, tlbm_stream_t stream);

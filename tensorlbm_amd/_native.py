@@ -122,6 +122,8 @@ _SIGNATURES = {
     "tlbm_cavities_layout": ([_VP, _I, _VP, _VP], _I),
     "tlbm_cavities_step": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _VP], _I),
     "tlbm_cavities_graph": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _I, _VP], _I),
+    "tlbm_cavities_step2": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _VP], _I),
+    "tlbm_cavities_graph2": ([_I, _VP, _VP, _I64, _VP, _I, ctypes.c_double, _I, _I, _VP], _I),
     "tlbm_graph_launch": ([_VP, _VP], _I),
     "tlbm_graph_destroy": ([_VP], _I),
 }

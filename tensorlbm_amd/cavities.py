@@ -6,9 +6,13 @@ reference's cavity chain (pull stream, Zou-He moving lid on y = L−1 from
 faces, BGK) on its own grid.  ``BatchedCavities`` packs all of them into one device buffer
 (cavity ``i`` at an element offset, ``[z][q][y][x]`` with a padded plane) and advances them with
 ``tlbm_cavities_step``; ``run(n, graph=True)`` replays a captured HIP graph of ping-pong steps.
+With ``temporal_blocking=True`` (fp32 or 16-bit deviation storage) ``run`` advances two steps per
+launch (``tlbm_cavities_step2``: the two-step march over 64 × 8 column tiles of every cavity, the
+intermediate lattice kept on chip).
 
-``autotune()`` picks the launch shape (one workgroup per 256-cell tile, or a persistent grid of
-a given size) by timing each candidate on the actual batch.
+``autotune()`` picks the launch shape by timing each candidate on the actual batch: one workgroup
+per 256-cell tile or a persistent grid of a given size for one step per launch, and, where the
+storage allows it, the z chunk of the two-step launch; it switches to the fastest.
 """
 
 from __future__ import annotations
@@ -55,7 +59,7 @@ class BatchedCavities:
 
     def __init__(self, edges: Sequence[int], taus: Sequence[float], *, u_lid: float = 0.1,
                  device: str | torch.device = "cuda:0", dtype: torch.dtype = torch.float32,
-                 deviation: bool = False) -> None:
+                 deviation: bool = False, temporal_blocking: bool = False) -> None:
         if len(edges) != len(taus) or not 1 <= len(edges) <= nat.CAVITIES_MAX:
             raise ValueError(f"need 1..{nat.CAVITIES_MAX} cavities with one tau each")
         self.device = torch.device(device)
@@ -79,6 +83,12 @@ class BatchedCavities:
         self.a = torch.zeros(self.total_elements, dtype=dtype, device=self.device)
         self.b = torch.zeros_like(self.a)
         self.grid = 0
+        # two steps per launch: fp32 or 16-bit deviation storage
+        self.two_step_capable = self._store in (nat.STORE_F32, nat.STORE_F16_DEV, nat.STORE_BF16_DEV)
+        if temporal_blocking and not self.two_step_capable:
+            raise ValueError("two steps per launch take float32 or 16-bit deviation storage")
+        self.two_step = bool(temporal_blocking)
+        self.zc = 0  # z planes per two-step workgroup; 0: the launch-shape model picks
         self._graphs: dict = {}
 
     # -- layout --------------------------------------------------------------
@@ -122,6 +132,30 @@ class BatchedCavities:
                                                nat.stream_of(self.a)), "tlbm_cavities_step")
         self.a, self.b = self.b, self.a
 
+    def step2(self) -> None:
+        """Two steps of every cavity: one launch with ``two_step``, else two."""
+        if not self.two_step:
+            self.step()
+            self.step()
+            return
+        nat.check(nat.lib().tlbm_cavities_step2(self._store, self.a.data_ptr(), self.b.data_ptr(), self.a.numel(),
+                                                self._table, self.n, self.u_lid, int(self.zc),
+                                                nat.stream_of(self.a)), "tlbm_cavities_step2")
+        self.a, self.b = self.b, self.a
+
+    def _graph2(self, launches: int):
+        key = ("step2", self.a.data_ptr(), self.b.data_ptr(), int(self.zc), int(launches))
+        g = self._graphs.get(key)
+        if g is None:
+            h = ctypes.c_void_p()
+            nat.check(nat.lib().tlbm_cavities_graph2(self._store, self.a.data_ptr(), self.b.data_ptr(),
+                                                     self.a.numel(), self._table, self.n, self.u_lid,
+                                                     int(self.zc), int(launches), ctypes.byref(h)),
+                      "tlbm_cavities_graph2")
+            g = h.value
+            self._graphs[key] = g
+        return g
+
     def _graph(self, steps: int):
         key = (self.a.data_ptr(), self.b.data_ptr(), int(self.grid), int(steps))
         g = self._graphs.get(key)
@@ -136,8 +170,26 @@ class BatchedCavities:
         return g
 
     def run(self, n_steps: int, *, graph: bool = True, graph_steps: int = 20) -> None:
-        """Advance ``n_steps``: whole graphs of ``graph_steps`` ping-pong steps, eager launches for the rest."""
+        """Advance ``n_steps``: whole graphs of ``graph_steps`` ping-pong steps, eager launches for the rest.
+
+        With ``two_step`` a graph holds ``graph_steps / 2`` two-step launches (``graph_steps`` a
+        multiple of 4) and the eager rest runs in pairs, an odd last step with one step per launch.
+        """
         n_steps = int(n_steps)
+        if self.two_step:
+            if graph and n_steps >= graph_steps:
+                if graph_steps < 4 or graph_steps % 4:
+                    raise ValueError("with two steps per launch graph_steps must be a multiple of 4")
+                g = self._graph2(graph_steps // 2)
+                s = nat.stream_of(self.a)
+                for _ in range(n_steps // graph_steps):
+                    nat.check(nat.lib().tlbm_graph_launch(g, s), "tlbm_graph_launch")
+                n_steps %= graph_steps
+            for _ in range(n_steps // 2):
+                self.step2()
+            if n_steps % 2:
+                self.step()
+            return
         if graph and n_steps >= graph_steps:
             if graph_steps < 2 or graph_steps % 2:
                 raise ValueError("graph_steps must be even and >= 2")
@@ -172,13 +224,32 @@ class BatchedCavities:
         end.synchronize()
         return start.elapsed_time(end) * 1e-3 / n_steps
 
-    def autotune(self, candidates: Sequence[int] | None = None, n_steps: int = 10) -> dict:
-        """Pick the fastest launch shape on this batch; returns {grid: seconds per step}."""
+    def autotune(self, candidates: Sequence[int] | None = None, n_steps: int = 10,
+                 zc_candidates: Sequence[int] | None = None) -> dict:
+        """Time every launch shape on this batch and keep the fastest; returns {shape: seconds per step}.
+
+        Shapes: ``"grid=G"`` -- one step per launch, G = 0 one workgroup per tile, else a persistent
+        grid of G workgroups (``candidates``); ``"zc=Z"`` -- two steps per launch with z chunks of Z
+        planes, Z = 0 the launch-shape model's choice (``zc_candidates``; only where the storage
+        allows two steps per launch).  Afterwards ``grid`` is the fastest one-step shape (used for
+        odd steps) and ``two_step`` / ``zc`` follow the fastest shape overall.
+        """
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        cands = list(candidates) if candidates is not None else [0] + [cus * k for k in (4, 8, 16, 32)]
-        res = {}
-        for g in cands:
+        grids = list(candidates) if candidates is not None else [0] + [cus * k for k in (4, 8, 16, 32)]
+        zcs = list(zc_candidates) if zc_candidates is not None else [0, 8, 16, 32, 64]
+        res, one = {}, {}
+        self.two_step = False
+        for g in grids:
             self.grid = int(g)
-            res[int(g)] = self.time_steps(n_steps, graph=False)
-        self.grid = min(res, key=res.get)
+            one[int(g)] = res[f"grid={int(g)}"] = self.time_steps(n_steps, graph=False)
+        self.grid = min(one, key=one.get)
+        if self.two_step_capable:
+            self.two_step = True
+            for z in zcs:
+                self.zc = int(z)
+                res[f"zc={int(z)}"] = self.time_steps(n_steps, graph=False)
+        best = min(res, key=res.get)
+        self.two_step = best.startswith("zc=")
+        if self.two_step:
+            self.zc = int(best.split("=")[1])
         return res

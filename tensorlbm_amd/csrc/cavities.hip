@@ -32,6 +32,9 @@
 namespace tlbm {
 
 int set_error(int code, const char* msg);  // tlbm.hip: shared tlbm_last_error() channel
+// tlbm.hip: two steps of every cavity per launch (k_cav2), table already validated
+int cav2_step(int store, const void* fin, void* fout, const tlbm_cavity* cavs, int n, double u_lid, int zc,
+              hipStream_t s);
 
 constexpr int CAV_TILE = 256;
 constexpr int CAV_MAX = TLBM_CAVITIES_MAX;
@@ -186,6 +189,47 @@ struct tlbm_graph {
   hipEvent_t before, after;     // orders the side stream against the null stream
 };
 
+namespace {
+
+// Capture `count` launches (launch(k, stream) for k = 0 .. count-1) into an
+// instantiated graph with its own fence stream and events.
+template <class Launch>
+int capture(int count, Launch launch, tlbm_graph** out) {
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return cfail(TLBM_ERR_HIP, "hipStreamCreate failed");
+  tlbm_graph* g = new (std::nothrow) tlbm_graph{nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (!g) {
+    (void)hipStreamDestroy(s);
+    return cfail(TLBM_ERR_HIP, "out of host memory");
+  }
+  int rc = TLBM_OK;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = cfail(TLBM_ERR_HIP, "hipStreamBeginCapture failed");
+  } else {
+    for (int k = 0; k < count && rc == TLBM_OK; ++k) rc = launch(k, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc == TLBM_OK && e != hipSuccess) rc = cfail(TLBM_ERR_HIP, "hipStreamEndCapture failed (%lld)", int(e));
+    g->graph = graph;
+    if (rc == TLBM_OK && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = cfail(TLBM_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  (void)hipStreamDestroy(s);
+  if (rc == TLBM_OK && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+                        hipEventCreateWithFlags(&g->before, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&g->after, hipEventDisableTiming) != hipSuccess))
+    rc = cfail(TLBM_ERR_HIP, "stream / event creation failed");
+  if (rc != TLBM_OK) {
+    tlbm_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return TLBM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int tlbm_cavities_layout(tlbm_cavity* cavs, int n, int64_t* total_elements, int64_t* total_tiles) {
@@ -223,37 +267,31 @@ int tlbm_cavities_graph(int store, void* f_a, void* f_b, int64_t capacity, const
   int rc = build_args(ab, store, f_a, f_b, cavs, n, capacity, u_lid);
   if (rc != TLBM_OK) return rc;
   build_args(ba, store, f_b, f_a, cavs, n, capacity, u_lid);
-  hipStream_t s;
-  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
-    return cfail(TLBM_ERR_HIP, "hipStreamCreate failed");
-  tlbm_graph* g = new (std::nothrow) tlbm_graph{nullptr, nullptr, nullptr, nullptr, nullptr};
-  if (!g) {
-    (void)hipStreamDestroy(s);
-    return cfail(TLBM_ERR_HIP, "out of host memory");
-  }
-  rc = TLBM_OK;
-  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-    rc = cfail(TLBM_ERR_HIP, "hipStreamBeginCapture failed");
-  } else {
-    for (int k = 0; k < steps && rc == TLBM_OK; ++k) rc = launch(k % 2 ? ba : ab, store, grid, s);
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &graph);
-    if (rc == TLBM_OK && e != hipSuccess) rc = cfail(TLBM_ERR_HIP, "hipStreamEndCapture failed (%lld)", int(e));
-    g->graph = graph;
-    if (rc == TLBM_OK && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) != hipSuccess)
-      rc = cfail(TLBM_ERR_HIP, "hipGraphInstantiate failed");
-  }
-  (void)hipStreamDestroy(s);
-  if (rc == TLBM_OK && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
-                        hipEventCreateWithFlags(&g->before, hipEventDisableTiming) != hipSuccess ||
-                        hipEventCreateWithFlags(&g->after, hipEventDisableTiming) != hipSuccess))
-    rc = cfail(TLBM_ERR_HIP, "stream / event creation failed");
-  if (rc != TLBM_OK) {
-    tlbm_graph_destroy(g);
-    return rc;
-  }
-  *out = g;
-  return TLBM_OK;
+  return capture(steps, [&](int k, hipStream_t s) { return launch(k % 2 ? ba : ab, store, grid, s); }, out);
+}
+
+int tlbm_cavities_step2(int store, const void* f_in, void* f_out, int64_t capacity, const tlbm_cavity* cavs, int n,
+                        double u_lid, int zc, tlbm_stream_t stream) {
+  CavArgs a;
+  const int rc = build_args(a, store, f_in, f_out, cavs, n, capacity, u_lid);
+  if (rc != TLBM_OK) return rc;
+  if (zc < 0) return cfail(TLBM_ERR_INVALID, "zc must be >= 0 (0: the launch-shape model picks it)");
+  return tlbm::cav2_step(store, f_in, f_out, cavs, n, u_lid, zc, (hipStream_t)stream);
+}
+
+int tlbm_cavities_graph2(int store, void* f_a, void* f_b, int64_t capacity, const tlbm_cavity* cavs, int n,
+                         double u_lid, int zc, int launches, tlbm_graph** out) {
+  if (!out) return cfail(TLBM_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  if (launches < 2 || launches % 2)
+    return cfail(TLBM_ERR_INVALID, "a graph holds an even number of launches >= 2 (%lld)", launches);
+  if (zc < 0) return cfail(TLBM_ERR_INVALID, "zc must be >= 0 (0: the launch-shape model picks it)");
+  CavArgs check;
+  const int rc = build_args(check, store, f_a, f_b, cavs, n, capacity, u_lid);
+  if (rc != TLBM_OK) return rc;
+  return capture(launches, [&](int k, hipStream_t s) {
+    return tlbm::cav2_step(store, k % 2 ? f_b : f_a, k % 2 ? f_a : f_b, cavs, n, u_lid, zc, s);
+  }, out);
 }
 
 int tlbm_graph_launch(tlbm_graph* g, tlbm_stream_t stream) {

@@ -467,8 +467,13 @@ struct ZRing {
 // values a second launch would read; step two pulls from LDS, so a cell's
 // populations cross HBM once per two steps plus the tile halo.  The step-one
 // loads of the next plane are issued before the LDS phase of the current one.
-template <class L, int S, int COLL, int FEAT, int BCK, int TX, int TY>
-__device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, int zc) {
+//
+// `tile` is the workgroup's column tile and z chunk (x tiles fastest).  BoxWalls:
+// the solid cells are the five stationary walls of a cubic lid-driven cavity
+// (x = 0, x = nx-1, y = 0, z = 0, z = nz-1), decided from the coordinates
+// instead of read from a mask.
+template <class L, int S, int COLL, int FEAT, int BCK, int TX, int TY, bool BoxWalls = false>
+__device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, int zc, uint32_t tile) {
   using ST = Store<S>;
   using T = typename ST::T;
   using C = typename ST::C;
@@ -490,7 +495,6 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   __shared__ C s_eq[kAll ? Q : 1];
   const int t = threadIdx.x;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t ntx = uint32_t((nx + TX - 1) / TX), nty = uint32_t((ny + TY - 1) / TY);
   const int x0 = int(tile % ntx) * TX, y0 = int((tile / ntx) % nty) * TY, z0 = int(tile / (ntx * nty)) * zc;
   const int zend = z0 + zc < nz ? z0 + zc : nz;
@@ -542,7 +546,9 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     const int pz = p < 0 ? p + nz : (p >= nz ? p - nz : p);
     // the mask byte first: the loads complete in order, so consuming it does not
     // wait for the population loads behind it
-    if constexpr (kAll) {
+    if constexpr (kAll && BoxWalls) {
+      pre_mask = gx == 0 || gx == nx - 1 || gy == 0 || pz == 0 || pz == nz - 1;
+    } else if constexpr (kAll) {
       if (a.solid != nullptr) pre_mask = a.solid[(uint32_t(pz) * uint32_t(ny) + uint32_t(gy)) * uint32_t(nx) + uint32_t(gx)];
     }
     gather<L, S>(pre, fin, a, gxn, gy, pz);
@@ -637,7 +643,7 @@ constexpr int tb_threads() {
 
 template <class L, int S, int COLL, int FEAT, int BCK, int TX, int TY>
 __global__ __launch_bounds__((tb_threads<TX, TY>())) void k_lbm2(const KArgs<typename Store<S>::C> a, int zc) {
-  lbm2_body<L, S, COLL, FEAT, BCK, TX, TY>(a, zc);
+  lbm2_body<L, S, COLL, FEAT, BCK, TX, TY>(a, zc, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // The learned-closure variants hold the matrix-core tiles next to the Q
@@ -1050,6 +1056,117 @@ inline int step2_zc(int cols, int nz, int cus) {
   return best;
 }
 
+// Batched lid-driven cavities, two steps per launch: the two-step march over the
+// column tiles of every cavity in one grid.  Cavity i (edge E, [z][q][y][x] at
+// element `offset` with the (y, x) plane padded to a multiple of 64) owns
+// workgroups tiles_before .. tiles_before + ceil(E/TX) ceil(E/TY) ceil(E/zc) - 1;
+// a workgroup finds its cavity by a binary search over those prefix sums and
+// runs the single-grid march with the cavity's strides, relaxation rate, the
+// Zou-He lid and box walls -- per cavity what tlbm_step2 computes for that
+// cavity alone with its wall mask and TLBM_BC_LID_Y.
+struct Cav2Dev {
+  int64_t offset;
+  uint32_t tiles_before;
+  int32_t edge;
+  double omega;
+};
+
+template <typename C>
+struct Cav2Args {
+  KArgs<C> k;  // fields shared by every cavity (buffers, lid, collision mode)
+  uint32_t total;
+  int n, zc;
+  Cav2Dev c[TLBM_CAVITIES_MAX];
+};
+static_assert(sizeof(Cav2Args<float>) < 4000, "cavity table travels in the kernel arguments");
+
+template <int S>
+__global__ __launch_bounds__((tb_threads<Tile2<D3Q19, S>::TX, Tile2<D3Q19, S>::TY>())) void k_cav2(
+    const Cav2Args<typename Store<S>::C> a) {
+  using T = typename Store<S>::T;
+  using C = typename Store<S>::C;
+  const uint32_t g = xcd_tile(blockIdx.x, gridDim.x);
+  int lo = 0, hi = a.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.c[mid].tiles_before <= g) lo = mid; else hi = mid - 1;
+  }
+  const int E = a.c[lo].edge;
+  const int64_t plane = (int64_t(E) * E + 63) / 64 * 64;
+  KArgs<C> k = a.k;
+  k.fin = static_cast<const T*>(a.k.fin) + a.c[lo].offset;
+  k.fout = static_cast<T*>(a.k.fout) + a.c[lo].offset;
+  k.nx = k.ny = k.nz = E;
+  k.isq = k.osq = plane;
+  k.isz = k.osz = D3Q19::Q * plane;
+  k.isy = k.osy = E;
+  k.omega = C(a.c[lo].omega);
+  k.tau = C(1.0 / a.c[lo].omega);
+  lbm2_body<D3Q19, S, TLBM_COLL_BGK, FEAT_BC, TLBM_BC_LID_Y, Tile2<D3Q19, S>::TX, Tile2<D3Q19, S>::TY, true>(
+      k, a.zc, g - a.c[lo].tiles_before);
+}
+
+// Workgroups of a two-step cavity launch with z chunks of zc planes, per cavity
+// prefix sums into `a`; -1 when the count would not fit 31 bits.
+template <typename C, int TX, int TY>
+int64_t cav2_layout(Cav2Args<C>& a, const tlbm_cavity* cavs, int n, int zc) {
+  uint64_t w = 0;
+  for (int i = 0; i < n; ++i) {
+    const int E = int(cavs[i].edge);
+    a.c[i].offset = cavs[i].offset;
+    a.c[i].tiles_before = uint32_t(w);
+    a.c[i].edge = E;
+    a.c[i].omega = 1.0 / cavs[i].tau;
+    w += uint64_t((E + TX - 1) / TX) * uint64_t((E + TY - 1) / TY) * uint64_t((E + zc - 1) / zc);
+    if (w >= (uint64_t(1) << 31)) return -1;
+  }
+  return int64_t(w);
+}
+
+// z planes per workgroup for a batch of cavities: the rounds x planes model of
+// step2_zc over the workgroups of all cavities together
+template <int TX, int TY>
+int cav2_zc(const tlbm_cavity* cavs, int n, int cus) {
+  int emax = 1;
+  for (int i = 0; i < n; ++i) emax = cavs[i].edge > emax ? int(cavs[i].edge) : emax;
+  int best = emax;
+  int64_t best_cost = INT64_MAX;
+  for (int zc = 1; zc <= emax; ++zc) {
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+      const int E = int(cavs[i].edge);
+      w += int64_t((E + TX - 1) / TX) * ((E + TY - 1) / TY) * ((E + zc - 1) / zc);
+    }
+    const int64_t cost = (w + cus - 1) / cus * (zc + 2);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  return best;
+}
+
+template <int S>
+int launch_cav2(const void* fin, void* fout, const tlbm_cavity* cavs, int n, double u_lid, int zc, hipStream_t s) {
+  using C = typename Store<S>::C;
+  constexpr int TX = Tile2<D3Q19, S>::TX, TY = Tile2<D3Q19, S>::TY;
+  Cav2Args<C> a;
+  memset(&a, 0, sizeof(a));
+  a.k.fin = fin;
+  a.k.fout = fout;
+  a.k.tau_mode = TLBM_TAU_CONST;
+  a.k.eq_floor = 1;
+  a.k.bc_kind = TLBM_BC_LID_Y;
+  a.k.bc_u[0] = C(u_lid);
+  a.n = n;
+  a.zc = zc > 0 ? zc : cav2_zc<TX, TY>(cavs, n, device_cus());
+  const int64_t w = cav2_layout<C, TX, TY>(a, cavs, n, a.zc);
+  if (w < 0) return fail(TLBM_ERR_SHAPE, "too many workgroups for one launch");
+  a.total = uint32_t(w);
+  hipLaunchKernelGGL(k_cav2<S>, dim3(a.total), dim3(tb_threads<TX, TY>()), 0, s, a);
+  return check_launch("k_cav2");
+}
+
 // why a two-step launch cannot serve these arguments (nullptr: it can)
 inline const char* step2_reason(int lattice, int store, const tlbm_collision* coll, const tlbm_boundary* bc, int nx) {
   if (lattice == TLBM_D2Q9) return "two-step launches are built for D3Q19 and D3Q27";
@@ -1265,6 +1382,19 @@ int prepare(int lattice, int store, const void* f_in, const int64_t* in_strides,
 }
 
 }  // namespace
+
+namespace tlbm {
+// two steps of every cavity of a validated batch (cavities.hip checks the table)
+int cav2_step(int store, const void* fin, void* fout, const tlbm_cavity* cavs, int n, double u_lid, int zc,
+              hipStream_t s) {
+  switch (store) {
+    case TLBM_STORE_F32: return launch_cav2<ST_F32>(fin, fout, cavs, n, u_lid, zc, s);
+    case TLBM_STORE_F16_DEV: return launch_cav2<ST_F16_DEV>(fin, fout, cavs, n, u_lid, zc, s);
+    case TLBM_STORE_BF16_DEV: return launch_cav2<ST_BF16_DEV>(fin, fout, cavs, n, u_lid, zc, s);
+    default: return fail(TLBM_ERR_UNSUPPORTED, "two steps per launch take float32 or 16-bit deviation storage");
+  }
+}
+}  // namespace tlbm
 
 extern "C" {
 

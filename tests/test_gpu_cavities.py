@@ -160,3 +160,116 @@ def test_argument_errors(dev):
         BatchedCavities([8], [0.5], device=dev).step()
     with pytest.raises(ValueError):
         BatchedCavities([8] * 129, [0.6] * 129, device=dev)
+
+
+# -- two steps per launch (tlbm_cavities_step2) -----------------------------------------------
+
+EDGES2 = [8, 11, 13, 16, 5, 70]  # one x tile with clamped lanes .. a ragged second x tile
+TAUS2 = [0.8, 0.55, 0.62, 0.56, 1.3, 0.57]
+
+
+def _batch2(dev, dtype=torch.float32, deviation=False, seed=30, two_step=True):
+    b = BatchedCavities(EDGES2, TAUS2, u_lid=U_LID, device=dev, dtype=dtype, deviation=deviation,
+                        temporal_blocking=two_step)
+    states = []
+    for i, e in enumerate(EDGES2):
+        f0 = F.random_state(19, (e, e, e), seed=seed + i, noise=5e-3, u_amp=0.03)
+        states.append(f0)
+        w = ref.D3Q19.w.reshape(19, 1, 1, 1) if deviation else 0.0
+        b.cavity(i).copy_(torch.tensor(f0 - w, device=dev).to(dtype))
+    return b, states
+
+
+def test_two_step_batch_matches_oracle(dev):
+    """Two launches of two steps = four steps of the oracle's cavity chain (fp32 storage vs fp64)."""
+    b, states = _batch2(dev)
+    assert b.two_step
+    b.step2()
+    b.step2()
+    states = [ref.cavity_step(ref.cavity_step(ref.cavity_step(ref.cavity_step(s, t, U_LID), t, U_LID), t, U_LID),
+                              t, U_LID) for s, t in zip(states, TAUS2)]
+    for i in range(len(EDGES2)):
+        np.testing.assert_allclose(b.populations(i).cpu().numpy(), states[i], rtol=1e-5, atol=2e-7,
+                                   err_msg=f"cavity {i}")
+
+
+def test_two_step_matches_one_step_launches(dev):
+    """Same arithmetic in the one- and the two-step kernels, which are different code: instruction
+    selection may round a value differently by an ulp (6e-8 at f ~ 0.5) in a step, and such
+    differences accumulate over the 6 steps compared."""
+    b2, _ = _batch2(dev, seed=31)
+    b1, _ = _batch2(dev, seed=31, two_step=False)
+    for _ in range(3):
+        b2.step2()
+        b1.step2()  # two one-step launches
+    for i in range(len(EDGES2)):
+        d = float((b2.populations(i) - b1.populations(i)).abs().max())
+        assert d < 6 * 6e-8, (i, d)
+
+
+def test_two_step_z_chunks_agree_bit_for_bit(dev):
+    """The z chunk only changes which workgroup recomputes which halo plane, not any value."""
+    out = []
+    for zc in (0, 1, 3, 7):
+        b, _ = _batch2(dev, seed=32)
+        b.zc = zc
+        b.step2()
+        b.step2()
+        out.append([b.cavity(i).clone() for i in range(len(EDGES2))])
+    for other in out[1:]:
+        assert all(torch.equal(x, y) for x, y in zip(out[0], other))
+
+
+def test_two_step_graph_replays_match_eager_pairs(dev):
+    be, _ = _batch2(dev, seed=33)
+    bg, _ = _batch2(dev, seed=33)
+    for rep in range(3):
+        bg.run(8, graph=True, graph_steps=8)
+        be.run(8, graph=False)
+        torch.cuda.synchronize()
+        assert all(torch.equal(bg.cavity(i), be.cavity(i)) for i in range(len(EDGES2))), f"replay {rep}"
+        _ = (bg.a * 2.0).sum().item()  # unrelated eager work between replays
+    bg.run(11, graph=True, graph_steps=4)  # two graph launches, one eager pair, one single step
+    be.run(11, graph=False)
+    torch.cuda.synchronize()
+    assert all(torch.equal(bg.cavity(i), be.cavity(i)) for i in range(len(EDGES2)))
+    with pytest.raises(ValueError):
+        bg.run(12, graph=True, graph_steps=6)
+
+
+def test_two_step_deviation_storage_tracks_fp32(dev):
+    b32, _ = _batch2(dev, seed=34)
+    b16, _ = _batch2(dev, torch.float16, deviation=True, seed=34)
+    for _ in range(10):
+        b32.step2()
+        b16.step2()
+    for i in range(len(EDGES2)):
+        a, c = b16.populations(i).double(), b32.populations(i).double()
+        assert torch.isfinite(a).all()
+        assert float((a - c).abs().max()) < 5e-4, (i, float((a - c).abs().max()))
+
+
+def test_autotune_keeps_the_fastest_shape(dev):
+    b, _ = _batch2(dev, seed=35, two_step=False)
+    res = b.autotune(candidates=[0, 256], n_steps=4, zc_candidates=[0, 4])
+    assert set(res) == {"grid=0", "grid=256", "zc=0", "zc=4"}
+    best = min(res, key=res.get)
+    assert b.two_step == best.startswith("zc=")
+    assert b.grid in (0, 256)
+    b.run(6, graph=True, graph_steps=4)
+    torch.cuda.synchronize()
+    assert all(torch.isfinite(b.cavity(i)).all() for i in range(len(EDGES2)))
+
+
+def test_two_step_argument_errors(dev):
+    with pytest.raises(ValueError):
+        BatchedCavities([8], [0.6], device=dev, dtype=torch.float64, temporal_blocking=True)
+    from tensorlbm_amd import _native as nat
+
+    b = BatchedCavities([8], [0.6], device=dev)
+    rc = nat.lib().tlbm_cavities_step2(nat.store_code(torch.float64), b.a.data_ptr(), b.b.data_ptr(), b.a.numel(),
+                                       b._table, 1, 0.1, 0, None)
+    assert rc != 0
+    rc = nat.lib().tlbm_cavities_step2(b._store, b.a.data_ptr(), b.b.data_ptr(), b.a.numel(), b._table, 1, 0.1, -1,
+                                       None)
+    assert rc == nat.ERR_INVALID
