@@ -399,6 +399,51 @@ __device__ __forceinline__ void inverse(const C (&m)[27], C (&v)[27]) {
       v[IDX.pop[0][y][z]] = fma(C(-0.5), m1, half);
     }
 }
+// Raw moments of the second-order equilibrium w_q rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2)
+// in closed form.  D3Q27's weights are products of 1-D weights (2/3, 1/6, 1/6) whose
+// moments of degree 0..4 are 1, 0, 1/3, 0, 1/3, so with S(e) = M(e_x) M(e_y) M(e_z):
+//   m_e = rho [ (1 - 1.5 u^2) S(e) + 3 sum_d u_d S(e + d) + 4.5 sum_{d,f} u_d u_f S(e + d + f) ].
+constexpr double m1d(int n) { return n == 0 ? 1.0 : (n % 2 ? 0.0 : 1.0 / 3.0); }
+constexpr double s3(int a, int b, int c) { return m1d(a) * m1d(b) * m1d(c); }
+struct EqCoef {
+  double c0[27];     // of (1 - 1.5 u^2)
+  double c1[27][3];  // of u_d
+  double c2[27][6];  // of u_x^2, u_y^2, u_z^2, u_x u_y, u_x u_z, u_y u_z
+};
+constexpr EqCoef make_eq_coef() {
+  EqCoef t{};
+  constexpr int pd[6] = {0, 1, 2, 0, 0, 1}, pf[6] = {0, 1, 2, 1, 2, 2};
+  for (int i = 0; i < 27; ++i) {
+    const int a = DEG[i][0], b = DEG[i][1], c = DEG[i][2];
+    t.c0[i] = s3(a, b, c);
+    for (int d = 0; d < 3; ++d) t.c1[i][d] = 3.0 * s3(a + (d == 0), b + (d == 1), c + (d == 2));
+    for (int k = 0; k < 6; ++k) {
+      const int d = pd[k], f = pf[k];
+      t.c2[i][k] = (d == f ? 4.5 : 9.0) *
+                   s3(a + (d == 0) + (f == 0), b + (d == 1) + (f == 1), c + (d == 2) + (f == 2));
+    }
+  }
+  return t;
+}
+constexpr EqCoef EQ = make_eq_coef();
+
+template <typename C>
+__device__ __forceinline__ void eq_moments(C rho, C ux, C uy, C uz, C (&m)[27]) {
+  const C u[3] = {ux, uy, uz};
+  const C uu[6] = {ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+  const C base = C(1) - C(1.5) * (uu[0] + uu[1] + uu[2]);
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    C acc = EQ.c0[i] != 0.0 ? C(EQ.c0[i]) * base : C(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (EQ.c1[i][d] != 0.0) acc = fma(C(EQ.c1[i][d]), u[d], acc);
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (EQ.c2[i][k] != 0.0) acc = fma(C(EQ.c2[i][k]), uu[k], acc);
+    m[i] = rho * acc;
+  }
+}
 }  // namespace f27
 
 template <class L, typename C>
@@ -533,13 +578,14 @@ __device__ __forceinline__ void cumulant19(C (&f)[L::Q], const C (&feq)[L::Q], c
 // Geier 2015 cumulants on the full distribution, relaxed towards the
 // equilibrium's cumulants (cumulant.py:1050-1156).  rate = {omega_b,
 // omega_odd, omega_even}.
+// The equilibrium enters only through its raw moments, taken in closed form
+// (f27::eq_moments) for density rho_eq.
 template <typename C>
-__device__ __forceinline__ void geier27(C (&f)[27], const C (&feq)[27], const Macro<C>& mc, C omega,
-                                        const C* rate) {
+__device__ __forceinline__ void geier27(C (&f)[27], C rho_eq, const Macro<C>& mc, C omega, const C* rate) {
   using L = D3Q27;
   C m[27], me[27], kt[27], c[27], ce[27];
   moments<L>(f, m);
-  moments<L>(feq, me);
+  f27::eq_moments(rho_eq, mc.ux, mc.uy, mc.uz, me);
   to_central<L>(m, mc.ux, mc.uy, mc.uz);
   to_central<L>(me, mc.ux, mc.uy, mc.uz);
   // C = rho * C~(kappa / rho), component 0 carries rho

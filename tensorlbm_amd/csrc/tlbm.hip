@@ -294,14 +294,21 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       }
       bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, omega);
     } else {
+      // the Geier collision takes the equilibrium's moments in closed form and the
+      // LES stress from the moments of f, so it never forms f_eq
+      constexpr bool kFeq = COLL != TLBM_COLL_GEIER;
       C feq[Q];
-      equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
+      if constexpr (kFeq) equilibrium<L>(rho_eq, mc.ux, mc.uy, mc.uz, feq);
       C pi[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
       if (need_pi) {
-        C fneq[Q];
+        if constexpr (kFeq) {
+          C fneq[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
-        neq_stress<L>(fneq, pi);
+          for (int q = 0; q < Q; ++q) fneq[q] = f[q] - feq[q];
+          neq_stress<L>(fneq, pi);
+        } else {
+          neq_stress_moments<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
+        }
       }
       if (a.tau_mode == TLBM_TAU_SMAG) {
         omega = smagorinsky_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
@@ -318,7 +325,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
         if constexpr (Q == 19) cumulant19<L>(f, feq, pi, omega, a.rate);
         if constexpr (Q == 27) legacy27<C>(f, feq, mc, omega, a.rate);
       } else if constexpr (COLL == TLBM_COLL_GEIER) {
-        if constexpr (Q == 27) geier27<C>(f, feq, mc, omega, a.rate);
+        if constexpr (Q == 27) geier27<C>(f, rho_eq, mc, omega, a.rate);
       }
     }
   }
