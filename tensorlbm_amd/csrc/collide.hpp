@@ -444,12 +444,58 @@ __device__ __forceinline__ void eq_moments(C rho, C ux, C uy, C uz, C (&m)[27]) 
     m[i] = rho * acc;
   }
 }
+// D3Q19 through the same sweeps: its velocities are D3Q27's without the eight
+// corners and its moments a subset of D3Q27's exponents.  A distribution without
+// corner populations has zero moments wherever all three exponents are nonzero
+// (only corners have three nonzero components), so its D3Q27 moments are its
+// D3Q19 moments plus those zeros, and the D3Q27 inverse of that vector is the
+// distribution itself, corners zero.  Exponent order: the CM19 rows.
+constexpr int DEG19[19][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2},
+                              {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1},
+                              {1, 0, 2}, {0, 1, 2}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}};
+constexpr bool matches_table19() {
+  for (int i = 0; i < 19; ++i)
+    for (int q = 0; q < 19; ++q)
+      if (tables::CM19[i][q] != ipow(D3Q19::cx[q], DEG19[i][0]) * ipow(D3Q19::cy[q], DEG19[i][1]) *
+                                   ipow(D3Q19::cz[q], DEG19[i][2]))
+        return false;
+  return true;
+}
+static_assert(matches_table19(), "D3Q19 moment exponents must follow the CM19 rows");
+
+template <typename C>
+__device__ __forceinline__ void forward19(const C (&v)[19], C (&m)[19]) {
+  C v27[27], m27[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v27[q] = C(0);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) v27[IDX.pop[D3Q19::cx[q] + 1][D3Q19::cy[q] + 1][D3Q19::cz[q] + 1]] = v[q];
+  forward(v27, m27);
+#pragma unroll
+  for (int i = 0; i < 19; ++i) m[i] = m27[IDX.mom[DEG19[i][0]][DEG19[i][1]][DEG19[i][2]]];
+}
+
+template <typename C>
+__device__ __forceinline__ void inverse19(const C (&m)[19], C (&v)[19]) {
+  C m27[27], v27[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) m27[i] = C(0);
+#pragma unroll
+  for (int i = 0; i < 19; ++i) m27[IDX.mom[DEG19[i][0]][DEG19[i][1]][DEG19[i][2]]] = m[i];
+  inverse(m27, v27);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) v[q] = v27[IDX.pop[D3Q19::cx[q] + 1][D3Q19::cy[q] + 1][D3Q19::cz[q] + 1]];
+}
 }  // namespace f27
 
 template <class L, typename C>
 __device__ __forceinline__ void moments(const C (&v)[L::Q], C (&m)[L::Q]) {
   if constexpr (L::Q == 27) {
     f27::forward(v, m);
+    return;
+  }
+  if constexpr (L::Q == 19) {
+    f27::forward19(v, m);
     return;
   }
 #pragma unroll
@@ -466,6 +512,10 @@ template <class L, typename C>
 __device__ __forceinline__ void populations(const C (&m)[L::Q], C (&v)[L::Q]) {
   if constexpr (L::Q == 27) {
     f27::inverse(m, v);
+    return;
+  }
+  if constexpr (L::Q == 19) {
+    f27::inverse19(m, v);
     return;
   }
 #pragma unroll
