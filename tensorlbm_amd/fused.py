@@ -242,10 +242,10 @@ class FusedSolver3D:
     results as single steps, one HBM round trip of the populations per two
     steps.  ``step2()`` is one such pair.  ``None`` (default) turns it on for
     D3Q19 with fp32 storage, where it pays on MI355X (256³ channel + sphere,
-    GLUPS one / two steps per launch: BGK 38.9 / 68.3, MRT 39.8 / 52.6, CM
-    39.6 / 51.0, cumulant 39.1 / 57.8); with 16-bit storage (fp16 75.3 / 73.8)
-    and for D3Q27 the recomputed tile halo costs more instruction issue than
-    the saved traffic.
+    GLUPS one / two steps per launch: BGK 39.0 / 68.2, MRT 39.7 / 54.5, CM
+    39.5 / 55.1, cumulant 39.1 / 56.7); with 16-bit storage (fp16 76.1 / 74.8)
+    and for D3Q27 (Geier fp32 25.0 / 24.1) the recomputed tile halo costs more
+    instruction issue than the saved traffic.
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
