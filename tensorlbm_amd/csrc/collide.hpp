@@ -185,16 +185,28 @@ struct MrtTab<D2Q9> {
   static constexpr const double (&M)[9][9] = tables::MRT9;
   static constexpr const double (&Mi)[9][9] = tables::MRT9_INV;
 };
+// D3Q19 / D3Q27 also carry the MRT basis as a combination of the monomial
+// moments: M = T CM and M^-1 = CM^-1 TI (CM: the monomial moment matrix).
 template <>
 struct MrtTab<D3Q19> {
   static constexpr const double (&M)[19][19] = tables::MRT19;
   static constexpr const double (&Mi)[19][19] = tables::MRT19_INV;
+  static constexpr const double (&T)[19][19] = tables::MRT19_T;
+  static constexpr const double (&TI)[19][19] = tables::MRT19_TI;
 };
 template <>
 struct MrtTab<D3Q27> {
   static constexpr const double (&M)[27][27] = tables::MRT27;
   static constexpr const double (&Mi)[27][27] = tables::MRT27_INV;
+  static constexpr const double (&T)[27][27] = tables::MRT27_T;
+  static constexpr const double (&TI)[27][27] = tables::MRT27_TI;
 };
+
+// monomial moments and their inverse (defined with the central-moment tables below)
+template <class L, typename C>
+__device__ __forceinline__ void moments(const C (&v)[L::Q], C (&m)[L::Q]);
+template <class L, typename C>
+__device__ __forceinline__ void populations(const C (&m)[L::Q], C (&v)[L::Q]);
 
 // f* = f - M^-1 S M (f - f_eq); S holds the per-moment rates, with the shear
 // entries given by omega (may vary per cell).  solver3d.py:107-160.
@@ -222,9 +234,49 @@ constexpr int mrt_group(int i) {
 
 // rate = (s_e, s_eps, s_q, s_pi) of the collision parameters
 template <class L, typename C>
+__device__ __forceinline__ C mrt_rate(int i, const C* rate, C omega) {
+  const int g = mrt_group<L>(i);
+  return g == MRT_SE     ? rate[0]
+         : g == MRT_SEPS ? rate[1]
+         : g == MRT_SQ   ? rate[2]
+         : g == MRT_SPI  ? rate[3]
+         : g == MRT_ONE  ? C(1)
+                         : omega;
+}
+
+template <class L, typename C>
 __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C* rate, C omega) {
   constexpr int Q = L::Q;
   C m[Q];
+  if constexpr (Q == 19 || Q == 27) {
+    // through the monomial moments: axis-factored transforms and the sparse T / TI
+    C v[Q], mono[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = f[q] - feq[q];
+    moments<L>(v, mono);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      m[i] = C(0);
+      if (mrt_group<L>(i) == MRT_CONSERVED) continue;  // relaxed at rate 0: never formed
+      C acc = C(0);
+#pragma unroll
+      for (int k = 0; k < Q; ++k)
+        if (MrtTab<L>::T[i][k] != 0.0) acc += C(MrtTab<L>::T[i][k]) * mono[k];
+      m[i] = acc * mrt_rate<L>(i, rate, omega);
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      C acc = C(0);
+#pragma unroll
+      for (int i = 0; i < Q; ++i)
+        if (mrt_group<L>(i) != MRT_CONSERVED && MrtTab<L>::TI[k][i] != 0.0) acc += C(MrtTab<L>::TI[k][i]) * m[i];
+      mono[k] = acc;
+    }
+    populations<L>(mono, v);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] -= v[q];
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     m[i] = C(0);

@@ -184,6 +184,10 @@ def fmt(x: Fraction) -> str:
     return repr(float(x)) if x.denominator != 1 else f"{x.numerator}.0"
 
 
+def matmul(a, b):
+    return [[sum(a[i][k] * b[k][j] for k in range(len(b))) for j in range(len(b[0]))] for i in range(len(a))]
+
+
 def emit_matrix(name, m):
     rows = ",\n    ".join("{" + ", ".join(fmt(v) for v in r) + "}" for r in m)
     n = len(m)
@@ -233,6 +237,13 @@ def main():
         lines.append(emit_matrix(name, m))
         lines.append(emit_matrix(name + "_INV", inverse(m)))
         lines.append(emit_groups(name + "_SHIFT", shift_groups(deg)))
+    # MRT moments as combinations of the monomial moments (M = T CM, M^-1 = CM^-1 T^-1 with
+    # T^-1 = CM M^-1): the kernels reach the MRT basis through the axis-factored monomial
+    # transforms plus these sparse recombinations
+    for name, mrt, c, deg in (("MRT19", mrt_d3q19(), D3Q19_C, DEG19), ("MRT27", mrt_d3q27(), D3Q27_C, DEG27)):
+        cm = monomial(c, deg)
+        lines.append(emit_matrix(name + "_T", matmul(mrt, inverse(cm))))
+        lines.append(emit_matrix(name + "_TI", matmul(cm, inverse(mrt))))
     lines.append(emit_transform("central_to_cumulant27", "kt", "ct", False))
     lines.append(emit_transform("cumulant_to_central27", "ct", "kt", True))
     lines.append("}}  // namespace tlbm::tables")
