@@ -223,7 +223,8 @@ def emit_transform(fn, src, dst, inverse_dir):
     return "\n".join(out)
 
 
-def main():
+def render() -> str:
+    """The text of tables.inc."""
     lines = [
         "// Generated by gen_tables.py -- exact moment tables and cumulant transforms.",
         "#pragma once",
@@ -247,7 +248,11 @@ def main():
     lines.append(emit_transform("central_to_cumulant27", "kt", "ct", False))
     lines.append(emit_transform("cumulant_to_central27", "ct", "kt", True))
     lines.append("}}  // namespace tlbm::tables")
-    Path(__file__).with_name("tables.inc").write_text("\n".join(lines) + "\n")
+    return "\n".join(lines) + "\n"
+
+
+def main():
+    Path(__file__).with_name("tables.inc").write_text(render())
 
 
 if __name__ == "__main__":
