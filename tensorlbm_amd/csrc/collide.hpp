@@ -680,16 +680,49 @@ __device__ __forceinline__ void cumulant19(C (&f)[L::Q], const C (&feq)[L::Q], c
 // Geier 2015 cumulants on the full distribution, relaxed towards the
 // equilibrium's cumulants (cumulant.py:1050-1156).  rate = {omega_b,
 // omega_odd, omega_even}.
-// The equilibrium enters only through its raw moments, taken in closed form
-// (f27::eq_moments) for density rho_eq.
+// Cumulants C = rho C~(kappa / rho) of the second-order equilibrium at density rho,
+// in the DEG27 order, in closed form: its raw moments (f27::eq_moments), shifted
+// to central moments and mapped by the set-partition formula, reduce to
+//   C_200 = rho/3,  C_110 = 0,  C_210 = -rho ux^2 uy,  C_111 = -rho ux uy uz,
+//   C_220 = 3 rho ux^2 uy^2,  C_211 = 3 rho ux^2 uy uz,  C_221 = -6 rho ux^2 uy^2 uz,
+//   C_222 = 0  (and permutations), i.e. the product of that pipeline, which they
+// replace (checked against it in floating point and symbolically in the tests).
+template <typename C>
+__device__ __forceinline__ void eq_cumulants27(C rho, C ux, C uy, C uz, C (&ce)[27]) {
+  const C rx2 = rho * ux * ux, ry2 = rho * uy * uy, rz2 = rho * uz * uz, rxyz = rho * ux * uy * uz;
+  const C r3 = rho * C(1.0 / 3.0);
+  ce[0] = rho;
+  ce[1] = ce[2] = ce[3] = C(0);
+  ce[4] = ce[5] = ce[6] = r3;
+  ce[7] = ce[8] = ce[9] = C(0);
+  ce[10] = -rx2 * uy;  // 210
+  ce[11] = -rx2 * uz;  // 201
+  ce[12] = -ry2 * ux;  // 120
+  ce[13] = -ry2 * uz;  // 021
+  ce[14] = -rz2 * ux;  // 102
+  ce[15] = -rz2 * uy;  // 012
+  ce[16] = -rxyz;      // 111
+  ce[17] = C(3) * rx2 * uy * uy;  // 220
+  ce[18] = C(3) * rx2 * uz * uz;  // 202
+  ce[19] = C(3) * ry2 * uz * uz;  // 022
+  ce[20] = C(3) * rxyz * ux;      // 211
+  ce[21] = C(3) * rxyz * uy;      // 121
+  ce[22] = C(3) * rxyz * uz;      // 112
+  ce[23] = C(-6) * rxyz * ux * uy;  // 221
+  ce[24] = C(-6) * rxyz * ux * uz;  // 212
+  ce[25] = C(-6) * rxyz * uy * uz;  // 122
+  ce[26] = C(0);                    // 222
+}
+
+// The equilibrium enters only through its cumulants, taken in closed form
+// (eq_cumulants27) for density rho_eq.
 template <typename C>
 __device__ __forceinline__ void geier27(C (&f)[27], C rho_eq, const Macro<C>& mc, C omega, const C* rate) {
   using L = D3Q27;
-  C m[27], me[27], kt[27], c[27], ce[27];
+  C m[27], kt[27], c[27], ce[27];
   moments<L>(f, m);
-  f27::eq_moments(rho_eq, mc.ux, mc.uy, mc.uz, me);
   to_central<L>(m, mc.ux, mc.uy, mc.uz);
-  to_central<L>(me, mc.ux, mc.uy, mc.uz);
+  eq_cumulants27(rho_eq, mc.ux, mc.uy, mc.uz, ce);
   // C = rho * C~(kappa / rho), component 0 carries rho
   {
     const C r = m[0], ir = C(1) / r;
@@ -699,15 +732,6 @@ __device__ __forceinline__ void geier27(C (&f)[27], C rho_eq, const Macro<C>& mc
 #pragma unroll
     for (int i = 0; i < 27; ++i) c[i] *= r;
     c[0] = r;
-  }
-  {
-    const C r = me[0], ir = C(1) / r;
-#pragma unroll
-    for (int i = 0; i < 27; ++i) kt[i] = me[i] * ir;
-    tables::central_to_cumulant27(kt, ce);
-#pragma unroll
-    for (int i = 0; i < 27; ++i) ce[i] *= r;
-    ce[0] = r;
   }
   const C ob = rate[0], oo = rate[1], oe = rate[2];
   const C d4 = c[4] - ce[4], d5 = c[5] - ce[5], d6 = c[6] - ce[6];
