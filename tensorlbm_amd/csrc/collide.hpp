@@ -668,13 +668,14 @@ __device__ __forceinline__ void cumulant19(C (&f)[L::Q], const C (&feq)[L::Q], c
   ps[3] = pi[3] - omega * pi[3];
   ps[4] = pi[4] - omega * pi[4];
   ps[5] = pi[5] - omega * pi[5];
+  // f_eq + H(ps) + (1 - oe) (f - f_eq - H(pi)) with H linear in its tensor:
+  // f + oe (f_eq - f) + H(ps - (1 - oe) pi), one Hermite contraction per population
+  const C keep = C(1) - oe;
+  C d[6];
 #pragma unroll
-  for (int q = 0; q < L::Q; ++q) {
-    const C fneq = f[q] - feq[q];
-    const C freg = hermite2<L>(q, ps);
-    const C fho = fneq - hermite2<L>(q, pi);
-    f[q] = feq[q] + freg + (C(1) - oe) * fho;
-  }
+  for (int k = 0; k < 6; ++k) d[k] = ps[k] - keep * pi[k];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) f[q] = fma(oe, feq[q] - f[q], f[q]) + hermite2<L>(q, d);
 }
 
 // Geier 2015 cumulants on the full distribution, relaxed towards the

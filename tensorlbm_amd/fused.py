@@ -242,10 +242,10 @@ class FusedSolver3D:
     results as single steps, one HBM round trip of the populations per two
     steps.  ``step2()`` is one such pair.  ``None`` (default) turns it on for
     D3Q19 with fp32 storage, where it pays on MI355X (256³ channel + sphere,
-    GLUPS one / two steps per launch: BGK 39.0 / 68.2, MRT 39.7 / 54.5, CM
-    39.5 / 55.1, cumulant 39.1 / 56.7); with 16-bit storage (fp16 76.1 / 74.8)
-    and for D3Q27 (Geier fp32 25.0 / 24.1) the recomputed tile halo costs more
-    instruction issue than the saved traffic.
+    GLUPS one / two steps per launch: BGK 39.0 / 68.2, MRT 39.6 / 58.2, CM
+    39.7 / 55.4, cumulant 39.1 / 59.1); with 16-bit storage (fp16 76.1 / 74.6)
+    and for D3Q27 (Geier fp32 27.5 / 28.4, fp16 40.8 / 35.6) the recomputed tile
+    halo costs about what the saved traffic gains, or more.
     """
 
     def __init__(self, nz: int, ny: int, nx: int, tau: float, device: str | torch.device = "cuda:0",
