@@ -61,3 +61,34 @@ def test_fp16_storage_with_learned_closure_vs_fp64(dev):
     # activation rounding flips between storage precisions stay far below the fp16-store error
     assert e32["u"] < 3e-4
     assert e16["u"] < 1e-3
+
+
+def test_dhit_256_geier_per_field_errors_vs_fp64(dev):
+    """The configuration at full size: 256³ decaying turbulence (k0 = 4, u_rms = 0.03), D3Q27 Geier cumulants,
+    τ = 0.505, 200 steps; fp32 and 16-bit deviation storage against an FP64 run, per field."""
+    n, steps = 256, 200
+    u = fields.random_solenoidal_field(n, 4.0, 0.03, seed=11, device=dev, dtype=torch.float64)
+    f0 = tl.equilibrium27(torch.ones((n, n, n), device=dev, dtype=torch.float64), u[0].contiguous(),
+                          u[1].contiguous(), u[2].contiguous())
+    del u
+    w = tl.lattice.D3Q27.W_EXACT64.to(dev).reshape(27, 1, 1, 1)
+
+    def run(dtype, deviation):
+        s = tl.FusedSolver3D(n, n, n, 0.505, dev, dtype, lattice="D3Q27", collision="GEIER", deviation=deviation)
+        f, _ = s.run((f0 - w if deviation else f0).to(dtype), steps)
+        out = f.contiguous()
+        del s, f
+        return out
+
+    ref = run(torch.float64, False)
+    errs = {}
+    for name, dtype, dv in (("fp32", torch.float32, False), ("fp16dev", torch.float16, True),
+                            ("bf16dev", torch.bfloat16, True)):
+        got = run(dtype, dv)
+        errs[name] = dg.field_errors(got, ref, deviation=dv)
+        del got
+        torch.cuda.empty_cache()
+    print({k: {f: f"{v:.2e}" for f, v in e.items()} for k, e in errs.items()})
+    assert errs["fp32"]["u"] < 5e-5 and errs["fp32"]["pi"] < 5e-4
+    assert errs["fp16dev"]["u"] < 1e-3 and errs["fp16dev"]["pi"] < 5e-3
+    assert errs["bf16dev"]["u"] < 1e-2 and errs["bf16dev"]["pi"] < 5e-2
