@@ -232,8 +232,10 @@ class BatchedCavities:
         grid of G workgroups (``candidates``); ``"zc=Z"`` -- two steps per launch with z chunks of Z
         planes, Z = 0 the launch-shape model's choice (``zc_candidates``; only where the storage
         allows two steps per launch).  Afterwards ``grid`` is the fastest one-step shape (used for
-        odd steps) and ``two_step`` / ``zc`` follow the fastest shape overall.
+        odd steps) and ``two_step`` / ``zc`` follow the fastest shape overall.  The timed steps run on a
+        copy of the populations: the state is restored, so a run tuned first stays on its own timeline.
         """
+        saved = self.a.clone()
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         grids = list(candidates) if candidates is not None else [0] + [cus * k for k in (4, 8, 16, 32)]
         zcs = list(zc_candidates) if zc_candidates is not None else [0, 8, 16, 32, 64]
@@ -252,4 +254,6 @@ class BatchedCavities:
         self.two_step = best.startswith("zc=")
         if self.two_step:
             self.zc = int(best.split("=")[1])
+        self.a.copy_(saved)
+        del saved
         return res

@@ -251,7 +251,11 @@ def test_two_step_deviation_storage_tracks_fp32(dev):
 
 def test_autotune_keeps_the_fastest_shape(dev):
     b, _ = _batch2(dev, seed=35, two_step=False)
+    before = [b.populations(i).clone() for i in range(len(EDGES2))]
     res = b.autotune(candidates=[0, 256], n_steps=4, zc_candidates=[0, 4])
+    torch.cuda.synchronize()
+    # tuning times steps on a copy: the state it leaves is the state it found
+    assert all(torch.equal(b.populations(i), before[i]) for i in range(len(EDGES2)))
     assert set(res) == {"grid=0", "grid=256", "zc=0", "zc=4"}
     best = min(res, key=res.get)
     assert b.two_step == best.startswith("zc=")
