@@ -1012,7 +1012,10 @@ int launch_lbm_kind(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32
 // D3Q19 fp32: 150 KB for three planes) where three planes fit the 160 KB of LDS
 // next to the mask and f_eq tables, else 32 x 8 (6 waves; D3Q27 with 4-byte
 // storage).  Measured on the 256^3 channel: 64 x 6 46.0, 64 x 7 50.7, 64 x 8
-// 57.2 GLUPS (more waves per CU and less recomputed halo).
+// 57.2 GLUPS (more waves per CU and less recomputed halo).  32 x 16 recomputes
+// less halo (1.20x against 1.29x) and pads uneven grids less, yet ran slower:
+// 61.6 against 64.9 GLUPS on the channel and 34.6 against 42.2 on 64 cavities
+// of edges 32-160 -- rows of 64 keep each pulled row in whole cache lines.
 template <class L, int S>
 struct Tile2 {
   static constexpr bool wide = 3 * L::Q * 66 * 10 * sizeof(typename Store<S>::T) <= 158 * 1024;
