@@ -523,13 +523,15 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   const int gxn = nb_edge ? (gx == 0 ? 1 : nx - 2) : gx;
   // output cell of this lane.  The output lanes fill whole waves; lanes of a
   // ragged tile whose cell lies past the grid take the tile's last in-grid cell
-  // instead, compute that cell from the same inputs with the same code and store
-  // the same bits as its own lane, so the stores need no lane mask (see the loop)
+  // instead and compute it from the same inputs with the same code, so every
+  // wave runs the update unmasked; only lanes that own their cell store (`own`),
+  // so no cell is written twice.
   static_assert(TX * TY % 64 == 0, "output lanes fill whole waves");
   const bool out_wave = __builtin_amdgcn_readfirstlane(t) < TX * TY;
   const bool h2 = t < TX * TY;
   const int ux_ = min(x0 + (h2 ? t % TX : 0), nx - 1), uy_ = min(y0 + (h2 ? t / TX : 0), ny - 1);
   const int ox = ux_ - x0, oy = uy_ - y0;
+  const bool own = h2 && x0 + t % TX < nx && y0 + t / TX < ny;
   double unused_force = 0.0;  // no force sample here: tlbm_step2 takes no force buffer
 
   if constexpr (kAll) {
@@ -629,8 +631,10 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
                                                          unused_force, unused_force, lds_at, eq_unit);
       T* fout = static_cast<T*>(a.fout);
       const uint32_t off = uint32_t(uy_) * uint32_t(a.osy) + uint32_t(ux_);
+      if (own) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
+        for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
+      }
     } else {
       wait_vm_loads();
     }
