@@ -532,12 +532,75 @@ def hermite_reconstruct(lat: Lattice, pis, dtype) -> np.ndarray:
     return 4.5 * w * h
 
 
-def collide_cumulant_d3q19(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0, *, omega=None):
+def edge_derivative(field: np.ndarray, axis: int) -> np.ndarray:
+    """d/dx along ``axis``: centred inside, second-order one-sided on the two end planes; an axis of
+    length 2 gets the forward difference on both planes, of length 1 zero (turbulence.py:595-640)."""
+    n = field.shape[axis]
+    if n == 1:
+        return np.zeros_like(field)
+    a = np.moveaxis(field, axis, 0)
+    if n == 2:
+        d = np.broadcast_to(a[1] - a[0], a.shape).copy()
+        return np.moveaxis(d, 0, axis)
+    d = np.empty_like(a)
+    d[1:-1] = 0.5 * (a[2:] - a[:-2])
+    d[0] = 0.5 * (-3.0 * a[0] + 4.0 * a[1] - a[2])
+    d[-1] = 0.5 * (3.0 * a[-1] - 4.0 * a[-2] + a[-3])
+    return np.moveaxis(d, 0, axis)
+
+
+def velocity_gradient(ux, uy, uz):
+    """``g[i][j] = ∂u_i/∂x_j`` on a (nz, ny, nx) grid, x = axis 2 (turbulence.py:662-695)."""
+    return [[edge_derivative(u, ax) for ax in (2, 1, 0)] for u in (ux, uy, uz)]
+
+
+def wale_nu_t(ux, uy, uz, C_w: float):
+    """WALE eddy viscosity ``C_w² |S^d|³ / (|S|⁵ + |S^d|^{5/2} + 1e-30)`` with Δ = 1; S^d the
+    traceless symmetric part of g·g (turbulence.py:754-841)."""
+    g = velocity_gradient(ux, uy, uz)
+    g2 = [[sum(g[i][k] * g[k][j] for k in range(3)) for j in range(3)] for i in range(3)]
+    tr = g2[0][0] + g2[1][1] + g2[2][2]
+    sd_n2 = sum((g2[i][i] - tr / 3.0) ** 2 for i in range(3))
+    sd_n2 = sd_n2 + 2.0 * sum((0.5 * (g2[i][j] + g2[j][i])) ** 2 for i, j in ((0, 1), (0, 2), (1, 2)))
+    s_n2 = sum(g[i][i] ** 2 for i in range(3))
+    s_n2 = s_n2 + 2.0 * sum((0.5 * (g[i][j] + g[j][i])) ** 2 for i, j in ((0, 1), (0, 2), (1, 2)))
+    sd_n2 = np.maximum(sd_n2, 0.0)
+    s_n2 = np.maximum(s_n2, 0.0)
+    return (C_w * C_w) * sd_n2**1.5 / (s_n2**2.5 + sd_n2**1.25 + 1e-30)
+
+
+def vreman_nu_t(ux, uy, uz, C_v: float):
+    """Vreman (2004) eddy viscosity ``C_v √(max(B_β, 0) / (|α|² + 1e-30))``, β = αᵀα, α_ij = ∂u_i/∂x_j,
+    B_β the sum of β's principal 2×2 minors (turbulence.py:844-872)."""
+    g = velocity_gradient(ux, uy, uz)
+    beta = [[sum(g[k][i] * g[k][j] for k in range(3)) for j in range(3)] for i in range(3)]
+    a2 = sum(g[i][j] ** 2 for i in range(3) for j in range(3))
+    bb = sum(beta[i][i] * beta[j][j] - beta[i][j] ** 2 for i, j in ((0, 1), (0, 2), (1, 2)))
+    return C_v * np.sqrt(np.maximum(bb, 0.0) / (a2 + 1e-30))
+
+
+def gradient_closure_tau(f, tau: float, C_w: float = 0.0, C_v: float = 0.0, solid_mask=None,
+                         wall_velocity=(0.0, 0.0, 0.0)):
+    """Per-cell ``τ_eff = clip(τ + 3 ν_t, 0.5001, max(1, τ + 0.5))`` of the WALE or Vreman closure
+    (turbulence.py:875-888), from the macroscopic velocity with solid cells set to ``wall_velocity``
+    (cumulant.py:1263-1303)."""
+    _, ux, uy, uz = macroscopic(f)
+    if solid_mask is not None:
+        m = np.asarray(solid_mask, bool)
+        ux, uy, uz = (np.where(m, f.dtype.type(w), u) for u, w in zip((ux, uy, uz), wall_velocity))
+    nu = wale_nu_t(ux, uy, uz, C_w) if C_w > 0.0 else vreman_nu_t(ux, uy, uz, C_v)
+    return np.clip(tau + 3.0 * nu, TAU_EFF_MIN, tau_eff_max(tau)).astype(f.dtype)
+
+
+def collide_cumulant_d3q19(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0, C_w=0.0, C_v=0.0,
+                           solid_mask=None, wall_velocity=(0.0, 0.0, 0.0), *, omega=None):
     """D3Q19 Hermite-regularised "cumulant" (cumulant.py:1159-1377).
 
     With ``C_s > 0`` the closure is the UNCLAMPED Hou form of that function
-    (cumulant.py:1316-1321).  ``omega`` overrides with a per-cell rate (the
-    fused kernel's CUMULANT branch, triton_fused_obstacle.py:1428-1478).
+    (cumulant.py:1316-1321); ``C_w`` / ``C_v`` select the WALE / Vreman
+    gradient closures (cumulant.py:1263-1303), whose velocity field takes
+    ``wall_velocity`` on ``solid_mask``.  ``omega`` overrides with a per-cell
+    rate (the fused kernel's CUMULANT branch, triton_fused_obstacle.py:1428-1478).
     ``omega_odd`` is accepted and unused, as in the reference.
     """
     del omega_odd
@@ -546,6 +609,8 @@ def collide_cumulant_d3q19(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C
     feq = equilibrium(lat, rho, ux, uy, uz)
     fneq = f - feq
     pis = neq_stress(fneq)
+    if omega is None and (C_w > 0.0 or C_v > 0.0):
+        omega = 1.0 / gradient_closure_tau(f, tau, C_w, C_v, solid_mask, wall_velocity)
     if omega is None:
         if C_s > 0.0:
             pxx, pyy, pzz, pxy, pxz, pyz = pis
