@@ -226,7 +226,7 @@ class TestFusedAdapters:
                     boundary="channel_x")
         st = tl.fused_state_dict(s, _f(torch.float16), step=4)
         assert st["lattice"] == "D3Q19" and st["q"] == 19 and st["grid"] == (4, 5, 6) and st["step"] == 4
-        assert st["precision_policy"] == "FP32FP16" and st["storage_dtype"] == "torch.float16"
+        assert st["precision_policy"] == "FP32FP16" and st["storage_dtype"] == "torch.float16:f-w"
         assert st["dtype"] == "torch.float16" and st["storage_offset"] == "f-w"
         assert st["collision"] == "MRT" and st["boundary"] == "channel_x"
         assert tl.fused_state_dict(_solver(), _f())["storage_offset"] == "none"
@@ -311,6 +311,33 @@ class TestFusedAdapters:
     def test_envelope_keeps_the_offset(self, tmp_path):
         st = tl.fused_state_dict(_solver(dtype=torch.bfloat16, deviation=True), _f(torch.bfloat16), step=6)
         c = tl.load_solver_checkpoint(tl.save_solver_checkpoint(tmp_path / "d.ckpt", st))
-        assert c.state["storage_offset"] == "f-w" and c.state["storage_dtype"] == "torch.bfloat16"
+        assert c.state["storage_offset"] == "f-w" and c.state["storage_dtype"] == "torch.bfloat16:f-w"
         g = tl.fused_load_state_dict(_solver(dtype=torch.bfloat16, deviation=True), c.state)
         assert torch.equal(g, st["populations"])
+
+    def test_reference_loader_refuses_a_deviation_state(self):
+        """The reference's fused loader (checkpoint.py:613-642) checks only the precision tier and the
+        storage dtype string.  A deviation state must fail the second check there, so that f - w is
+        never read as f; a plain fp16 state from the same tier must pass both."""
+
+        def reference_checks(solver, state):
+            if state.get("precision_policy") != getattr(getattr(solver, "precision_policy", None), "name", None):
+                raise ck.CheckpointError("precision policy")
+            saved = state.get("storage_dtype") or state.get("dtype")
+            if str(saved) != str(solver.dtype):
+                raise ck.CheckpointError("storage dtype")
+
+        ref_solver = _solver(precision_policy=tl.PrecisionPolicy.FP32FP16, dtype=torch.float16)
+        dev = tl.fused_state_dict(_solver(precision_policy=tl.PrecisionPolicy.FP32FP16, dtype=torch.float16,
+                                          deviation=True), _f(torch.float16))
+        with pytest.raises(ck.CheckpointError, match="storage dtype"):
+            reference_checks(ref_solver, dev)
+        plain = tl.fused_state_dict(ref_solver, _f(torch.float16))
+        reference_checks(ref_solver, plain)
+        assert plain["storage_dtype"] == "torch.float16" and plain["storage_offset"] == "none"
+
+    def test_storage_dtype_and_offset_must_agree(self):
+        st = tl.fused_state_dict(_solver(dtype=torch.float16, deviation=True), _f(torch.float16))
+        st["storage_offset"] = "none"
+        with pytest.raises(ck.CheckpointError, match="names offset 'f-w'"):
+            tl.fused_load_state_dict(_solver(dtype=torch.float16, deviation=True), st)

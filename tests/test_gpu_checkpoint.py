@@ -69,7 +69,7 @@ def test_fp16_deviation_resume_is_bitwise_and_offset_enforced(tmp_path):
     f, _ = first.run(f0, 3)
     path = tl.save_solver_checkpoint(tmp_path / "dev.ckpt", first.state_dict(f))
     c = tl.load_solver_checkpoint(path)
-    assert c.state["storage_offset"] == "f-w" and c.state["storage_dtype"] == "torch.float16"
+    assert c.state["storage_offset"] == "f-w" and c.state["storage_dtype"] == "torch.float16:f-w"
     with pytest.raises(tl.CheckpointError, match="storage offset"):
         tl.FusedSolver3D(*shape, 0.6, DEV, torch.float16).load_state_dict(c.state)
 
