@@ -8,14 +8,16 @@ fused pull-stream + inlet/outlet + bounce-back + collide update; the solver
 advances two steps per launch (``k_lbm2``, intermediate lattice in LDS).
 
 Prints ONE JSON line.  ``value`` is MLUPS (lattice cells updated per second,
-1e6) with the populations resident in HBM.  ``roofline`` prices the
-algorithmic 2·Q·itemsize bytes per cell-update (152 B for D3Q19 fp32; one
-launch updates every cell twice) against the 8 TB/s HBM peak using the
-average launch duration from HIP events on the launch stream; ``traffic`` is
-the HBM bytes one launch actually moves (PMC), below the algorithmic figure
-because the intermediate lattice never leaves the chip.  ``forms`` reports the
-other storage/collision forms of the same grid; ``cpu_baseline`` times the C
-oracle on the host cores.
+1e6) with the populations resident in HBM; every form is timed as the median
+of 5 windows of ``--steps`` steps (HIP events on the launch stream), after a
+1 s clock warm-up and ``--warmup`` untimed steps.  ``roofline`` prices the
+least bytes one launch must move (each population read and written once, plus
+the mask byte) against the 8 TB/s HBM peak; ``step_equivalent_GBs`` is the
+rate at 2·Q·itemsize bytes per cell-update, which a launch that keeps its
+intermediate lattice on chip can exceed.  ``traffic`` is the HBM bytes one
+launch actually moves (PMC).  ``forms`` reports the other storage/collision
+forms of the same grid; ``cpu_baseline`` times the reference's CPU step chain
+restated in eager torch on the host cores.
 """
 
 from __future__ import annotations
@@ -44,7 +46,7 @@ def parse():
     p.add_argument("--n", type=int, default=256, help="cubic grid edge")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-forms", action="store_true", help="only the headline form")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--copy-probe", action="store_true", help="also time a plain device copy of the same bytes")
     p.add_argument("--skip", default="", help="comma-separated forms to leave out: cavities, dhit")
     p.add_argument("--one-step", action="store_true", help="headline form with one step per launch (k_lbm)")
@@ -70,58 +72,111 @@ def channel_case(n: int, dev, torch):
     return solid.to(torch.int8).contiguous(), f
 
 
-def time_steps(step, f, steps, warmup, torch):
-    """Average ms per step with HIP events on torch's current stream (the launch stream)."""
+REPS = 5  # timed windows per form; the reported time is their median
+
+
+def _median(v):
+    v = sorted(v)
+    k = len(v)
+    return v[k // 2] if k % 2 else 0.5 * (v[k // 2 - 1] + v[k // 2])
+
+
+def warm_clocks(step, f, seconds, torch):
+    """Launch the step for about ``seconds`` so the GPU leaves its idle clocks before any timing."""
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            f = step(f)
+        torch.cuda.synchronize()
+        n += 10
+    return f, n
+
+
+def time_steps(step, f, steps, warmup, torch, reps=REPS):
+    """Median over ``reps`` windows of ``steps`` launches of the ms per launch, HIP events on torch's
+    current stream (the launch stream), after ``warmup`` untimed launches."""
     for _ in range(warmup):
         f = step(f)
     torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    windows = []
     t0 = time.perf_counter()
-    start.record()
-    for _ in range(steps):
-        f = step(f)
-    stop.record()
-    torch.cuda.synchronize()
+    for _ in range(reps):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            f = step(f)
+        stop.record()
+        torch.cuda.synchronize()
+        windows.append(start.elapsed_time(stop) / steps)
     wall = time.perf_counter() - t0
-    return start.elapsed_time(stop) / steps, wall, f
+    return _median(windows), {"min": min(windows), "max": max(windows), "reps": reps, "wall_s": wall}, f
+
+
+def per_launch_times(step, f, launches, torch):
+    """ms of each of ``launches`` launches, one event pair each (the spread between launches)."""
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(launches)]
+    torch.cuda.synchronize()
+    for a, b in evs:
+        a.record()
+        f = step(f)
+        b.record()
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in evs], f
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(n: int, tau: float, seconds: float):
-    """C oracle (OpenMP, fp32) fused BGK step with the sphere bounce-back on the full grid."""
+    """The reference's CPU path, restated in eager torch (collide_bgk3d -> stream3d_roll -> channel
+    inlet/outlet -> obstacle bounce-back), timed on the full grid for a bounded number of steps."""
     import numpy as np
+    import torch
 
-    from oracle import cfast, ref
+    from oracle import ref, torch_cpu
 
-    if not cfast.available():
-        cfast.build()
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
     rng = np.random.default_rng(SEED)
-    a = np.arange(n, dtype=np.float32)
-    r = 24.0 * n / 256.0
-    solid = ref.sphere_mask(n, n, n, 64.0 * n / 256.0, n / 2, n / 2, r)
-    pert = rng.uniform(-1e-3, 1e-3, (3, n, n, n)).astype(np.float32)
-    ux = (0.05 + pert[0]).astype(np.float32)
+    solid = torch.from_numpy(ref.sphere_mask(n, n, n, 64.0 * n / 256.0, n / 2, n / 2, 24.0 * n / 256.0))
+    pert = torch.from_numpy(rng.uniform(-1e-3, 1e-3, (3, n, n, n)).astype(np.float32))
+    ux = 0.05 + pert[0]
     ux[solid] = 0.0
-    f = ref.equilibrium(ref.D3Q19, np.ones((n, n, n), np.float32), ux, pert[1], pert[2]).astype(np.float32)
-    del pert, ux, a
-    out = np.empty_like(f)
-    cfast.fused_bgk(f, tau, solid, out=out)  # warm-up (page faults)
+    f = torch_cpu.equilibrium3d(torch.ones((n, n, n)), ux, pert[1], pert[2])
+    del pert, ux
+    f = torch_cpu.channel_step(f, tau, 0.05, solid)  # warm-up (allocator, page faults)
     steps, t0 = 0, time.perf_counter()
+    times = []
     while True:
-        cfast.fused_bgk(f, tau, solid, out=out)
-        f, out = out, f
+        t1 = time.perf_counter()
+        f = torch_cpu.channel_step(f, tau, 0.05, solid)
+        times.append(time.perf_counter() - t1)
         steps += 1
         el = time.perf_counter() - t0
         if el > seconds or steps >= 200:
             break
-    mlups = steps * n**3 / el / 1e6
+    ms = _median(times) * 1e3
+    finite = bool(torch.isfinite(f[:, ::8, ::8, ::8]).all())
+    del f
     return {
-        "value": round(mlups, 2),
+        "value": round(n**3 / (ms * 1e-3) / 1e6, 2),
         "unit": "MLUPS",
-        "cores": cfast.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"{steps} steps of the C oracle fused D3Q19 BGK fp32 step with the sphere bounce-back on the "
-                  f"full {n}^3 grid (periodic; inlet/outlet planes omitted) in {el:.1f} s on "
-                  f"{cfast.num_threads()} OpenMP threads, {platform.processor() or platform.machine()}",
+        "sample": f"{steps} eager-torch steps (median {ms:.0f} ms) of the reference's CPU chain collide_bgk3d -> "
+                  f"stream3d_roll -> channel inlet/outlet -> obstacle bounce-back on the full {n}^3 fp32 channel + "
+                  f"sphere grid, {el:.1f} s on {threads} torch threads",
+        "cpu_model": cpu_model(),
+        "torch": torch.__version__,
+        "finite": finite,
     }
 
 
@@ -156,11 +211,12 @@ def dhit_forms(torch, dev, n, steps=100):
         f = (f0 - w).to(dtype) if dv else f0.clone()
         s = tl.FusedSolver3D(n, n, n, 0.505, dev, dtype, lattice="D3Q27", collision="GEIER", deviation=dv)
         per = 2 if s.two_step else 1
-        ms, _, f = time_steps(s.step2 if per == 2 else s.step, f, steps // per, 10, torch)
-        ms /= per
+        ms_launch, _, f = time_steps(s.step2 if per == 2 else s.step, f, steps // per, 10, torch)
+        ms = ms_launch / per
         bpc = 2 * 27 * torch.tensor([], dtype=dtype).element_size()
         out[name] = {"ms_per_step": round(ms, 5), "mlups": round(n**3 / (ms * 1e-3) / 1e6, 1), "bytes_per_cell": bpc,
-                     "hbm_frac": round(bpc * n**3 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                     "steps_per_launch": per,
+                     "hbm_frac": round(bpc * n**3 / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         del s, f
     del f0
     torch.cuda.empty_cache()
@@ -206,7 +262,9 @@ def cavities_form(torch, dev, steps=100):
             "ms_per_step_graph": round(ms_graph, 4), "mlups_graph": round(cells / (ms_graph * 1e-3) / 1e6, 1),
             "ms_per_step_eager": round(ms_eager, 4), "ms_per_step_64_sequential": round(ms_seq, 4),
             "mlups_64_sequential": round(cells / (ms_seq * 1e-3) / 1e6, 1),
-            "hbm_frac_graph": round(cells * 152 / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "finite": ok}
+            # least bytes per launch: every population read and written once (no mask: walls from coordinates)
+            "hbm_frac_graph": round(cells * 152 / (ms_graph * (2 if two_step else 1) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_equivalent_GBs_graph": round(cells * 152 / (ms_graph * 1e-3) / 1e9, 1), "finite": ok}
 
 
 def main():
@@ -233,7 +291,8 @@ def main():
 
     closure = None
 
-    def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True, mlp=False, two_step=None):
+    def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True, mlp=False, two_step=None,
+             spread=False):
         f = f0.to(dtype)
         if deviation:
             f = (f0 - tl.lattice.D3Q19.W_EXACT64.to(dev).view(19, 1, 1, 1)).to(dtype)
@@ -244,18 +303,33 @@ def main():
                                   temporal_blocking=two_step)
         per = 2 if solver.two_step else 1
         launch = solver.step2 if per == 2 else solver.step
-        ms_launch, wall, f = time_steps(launch, f, args.steps // per, max(1, args.warmup // per), torch)
+        if not warmed:
+            f, _ = warm_clocks(launch, f, 1.0, torch)
+            warmed.append(True)
+        ms_launch, win, f = time_steps(launch, f, max(1, args.steps // per), max(1, args.warmup // per), torch)
         ms = ms_launch / per
-        bpc = 2 * q * torch.tensor([], dtype=dtype).element_size()
-        gbs = bpc * cells / (ms * 1e-3) / 1e9
-        ok = bool(torch.isfinite(f[:, ::16, ::16, ::16].float()).all())
+        esize = torch.tensor([], dtype=dtype).element_size()
+        bpc = 2 * q * esize
+        # least bytes one launch must move: every population read and written once, plus the mask byte
+        least = (bpc + (1 if mask else 0)) * cells
+        res = {"ms_per_step": round(ms, 5), "mlups": round(cells / (ms * 1e-3) / 1e6, 1),
+               "steps_per_launch": per, "ms_per_launch": round(ms_launch, 5),
+               "ms_per_launch_window_min_max": [round(win["min"], 5), round(win["max"], 5)],
+               "bytes_per_cell_update": bpc, "least_bytes_per_launch": least,
+               "achieved_GBs": round(least / (ms_launch * 1e-3) / 1e9, 1),
+               "hbm_frac": round(least / (ms_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "step_equivalent_GBs": round(bpc * cells / (ms * 1e-3) / 1e9, 1),
+               "windows": win["reps"], "wall_s": round(win["wall_s"], 3)}
+        if spread:
+            each, f = per_launch_times(launch, f, max(5, args.steps // per), torch)
+            res["ms_per_launch_each_min_median_max"] = [round(min(each), 5), round(_median(each), 5),
+                                                       round(max(each), 5)]
+        res["finite"] = bool(torch.isfinite(f[:, ::16, ::16, ::16].float()).all())
         del solver, f
-        return {"ms_per_step": round(ms, 5), "mlups": round(cells / (ms * 1e-3) / 1e6, 1),
-                "steps_per_launch": per, "ms_per_launch": round(ms_launch, 5),
-                "bytes_per_cell": bpc, "achieved_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                "finite": ok, "wall_s": round(wall, 3)}
+        return res
 
-    head = form(torch.float32, two_step=False if args.one_step else None)
+    warmed = []
+    head = form(torch.float32, two_step=False if args.one_step else None, spread=True)
     forms = {}
     if not args.no_forms:
         forms["bgk_fp32_channel_sphere_one_step_per_launch"] = form(torch.float32, two_step=False)
@@ -298,6 +372,7 @@ def main():
     per = head["steps_per_launch"]
     kernel = ("k_lbm2<D3Q19, fp32, BGK, mask + channel, 64x8 tile> (two steps per launch)" if per == 2 else
               "k_lbm<D3Q19, fp32, BGK, stream, mask + boundaries>")
+    traffic = measured_traffic(n, per)
     line = {
         "metric": METRIC,
         "value": value,
@@ -321,22 +396,27 @@ def main():
             "storage": "fp32",
             "solid_cells": int(solid.sum().item()),
             "seed": SEED,
+            "timing": f"median of {REPS} windows of {args.steps} steps after {args.warmup} warm-up steps and a "
+                      "1 s clock warm-up; HIP events on the launch stream",
         },
         "roofline": {
             "bound": "hbm",
             "achieved": head["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(head["achieved_GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n, per),
-            "traffic_GBs": (round(measured_traffic(n, per) / (head["ms_per_launch"] * 1e-3) / 1e9, 1)
-                            if measured_traffic(n, per) else None),
+            "frac": head["hbm_frac"],
+            "traffic": traffic,
+            "traffic_GBs": round(traffic / (head["ms_per_launch"] * 1e-3) / 1e9, 1) if traffic else None,
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; " + str(
                 PMC_SUMMARY[per].relative_to(ROOT)),
-            "algorithmic_bytes_per_launch": head["bytes_per_cell"] * cells * per,
-            "bytes_per_cell_update": head["bytes_per_cell"],
+            "least_bytes_per_launch": head["least_bytes_per_launch"],
+            "least_bytes": "each population read and written once per launch (2 x 19 x 4 B per cell) plus the "
+                           "1-byte solid mask",
+            "step_equivalent_GBs": head["step_equivalent_GBs"],
+            "bytes_per_cell_update": head["bytes_per_cell_update"],
             "steps_per_launch": per,
             "ms_per_launch": head["ms_per_launch"],
+            "ms_per_launch_each_min_median_max": head["ms_per_launch_each_min_median_max"],
             "kernel": kernel,
         },
         "forms": forms,

@@ -65,6 +65,12 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
 #define TLBM_TAU_SMAG_RAW 3 /* unclamped Hou closure of cumulant.py:1316-1321 */
 #define TLBM_TAU_MLP 4   /* learned closure: tau_eff = max(tau + 3 nu_t, 0.501) (ai/inference.py:56-65),
                             nu_t from the tlbm_mlp_pack blob in coll->mlp (D3Q19 BGK) */
+#define TLBM_TAU_FIELD64 5 /* per-cell tau_eff array of float64 values (coll->tau_eff points to doubles);
+                              tlbm_collide only -- the fp64 form of TLBM_TAU_FIELD */
+
+/* gradient eddy-viscosity closures for tlbm_gradient_closure_tau */
+#define TLBM_SGS_WALE 1   /* _wale_nu_t_3d, turbulence.py:754-841 */
+#define TLBM_SGS_VREMAN 2 /* _vreman_nu_t_3d, turbulence.py:844-872 */
 
 /* boundary kinds for tlbm_step */
 #define TLBM_BC_NONE 0
@@ -95,7 +101,7 @@ typedef struct tlbm_collision {
                            (0: the molecular tau, _strain_from_fneq_3d, turbulence.py:1449-1480) */
   double rate[8];       /* MRT: s_e, s_eps, s_q, s_pi | CM: s_bulk, s_3, s_4, s_5, s_6 |
                            CUMULANT/GEIER: omega_b, omega_odd, omega_even */
-  const float* tau_eff; /* TLBM_TAU_FIELD: (nz,ny,nx) float32 */
+  const float* tau_eff; /* TLBM_TAU_FIELD: (nz,ny,nx) float32; TLBM_TAU_FIELD64: float64 (cast) */
   const void* mlp;      /* TLBM_TAU_MLP: device pointer to packed MLP parameters */
 } tlbm_collision;
 
@@ -189,6 +195,20 @@ int tlbm_obstacle_force(int lattice, int store, const void* f, const int64_t* st
 int tlbm_smagorinsky_tau_sum(int lattice, int store, const void* f, const int64_t* strides, int nz,
                              int ny, int nx, double tau, double cs_delta_sq, double* out,
                              tlbm_stream_t stream);
+
+/*
+ * Per-cell tau_eff of a velocity-gradient closure (the C_w / C_v branches of
+ * collide_cumulant_d3q19, cumulant.py:1263-1303): the macroscopic velocity (ux, uy, uz: contiguous
+ * (nz, ny, nx), float32 or float64 by dtype) takes wall_velocity[3] (host doubles; NULL = 0) on cells
+ * where the optional int8 mask `solid` is non-zero, is differentiated with centred differences inside
+ * and second-order one-sided differences on the faces (_velocity_gradients_3d, turbulence.py:595-695),
+ * and tau_out = clip(tau + 3 nu_t, 0.5001, max(1, tau + 0.5)) (_nu_t_to_tau_eff, turbulence.py:875-888)
+ * with nu_t of model TLBM_SGS_WALE (coef = C_w) or TLBM_SGS_VREMAN (coef = C_v); tau_out has dtype.
+ * Feed it to tlbm_collide / tlbm_step as TLBM_TAU_FIELD (float32) or TLBM_TAU_FIELD64.
+ */
+int tlbm_gradient_closure_tau(int dtype, const void* ux, const void* uy, const void* uz, const int8_t* solid,
+                              const double* wall_velocity, int nz, int ny, int nx, int model, double coef, double tau,
+                              void* tau_out, tlbm_stream_t stream);
 
 /* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
 int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx,

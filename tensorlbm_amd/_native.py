@@ -29,7 +29,8 @@ ERR_HIP = -100
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
 STORE_F32, STORE_F64, STORE_F16, STORE_BF16, STORE_F16_DEV, STORE_BF16_DEV = range(6)
 COLL_NONE, COLL_BGK, COLL_MRT, COLL_CM, COLL_CUMULANT, COLL_GEIER = range(6)
-TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP = range(5)
+TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP, TAU_FIELD64 = range(6)
+SGS_WALE, SGS_VREMAN = 1, 2
 BC_NONE, BC_CHANNEL_X, BC_LID_Y, BC_FAR_FIELD = range(4)
 
 
@@ -110,6 +111,8 @@ _SIGNATURES = {
     "tlbm_moment_sums": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
     "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_scale": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_gradient_closure_tau": ([_I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, ctypes.c_double, ctypes.c_double,
+                                   _VP, _VP], _I),
     "tlbm_smagorinsky_tau_sum": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.c_double, ctypes.c_double, _VP, _VP], _I),
     "tlbm_mlp_sizes": ([_VP, _VP, _VP, _VP], _I),
     "tlbm_mlp_pack": ([_VP, _VP, _VP], _I),

@@ -299,6 +299,9 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       } else if (a.tau_mode == TLBM_TAU_FIELD) {
         omega = C(1) / C(a.tau_eff[cell]);
       }
+      if constexpr (!STREAM) {
+        if (a.tau_mode == TLBM_TAU_FIELD64) omega = C(1) / C(reinterpret_cast<const double*>(a.tau_eff)[cell]);
+      }
       bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, omega);
     } else {
       // the Geier collision takes the equilibrium's moments in closed form and the
@@ -323,6 +326,9 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
         omega = smagorinsky_raw_omega(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq);
       } else if (a.tau_mode == TLBM_TAU_FIELD) {
         omega = C(1) / C(a.tau_eff[cell]);
+      }
+      if constexpr (!STREAM) {
+        if (a.tau_mode == TLBM_TAU_FIELD64) omega = C(1) / C(reinterpret_cast<const double*>(a.tau_eff)[cell]);
       }
       if constexpr (COLL == TLBM_COLL_MRT) {
         mrt<L>(f, feq, a.rate, omega);
@@ -723,6 +729,97 @@ __global__ __launch_bounds__(256) void k_bounce_back(const T* fin, T* fout, cons
 }
 
 // ---------------------------------------------------------------------------
+// gradient eddy-viscosity closures (WALE, Vreman) -> per-cell tau_eff
+// ---------------------------------------------------------------------------
+
+// velocity component of cell i, or the wall velocity on solid cells
+template <typename T>
+__device__ __forceinline__ T vel_at(const T* u, const int8_t* solid, T wall, uint64_t i) {
+  return (solid != nullptr && solid[i] != 0) ? wall : u[i];
+}
+
+// d/dx along one axis of length n at position p (element stride st): centred inside,
+// second-order one-sided on the end planes, forward difference on an axis of length 2,
+// zero on an axis of length 1 (turbulence.py:595-640)
+template <typename T>
+__device__ __forceinline__ T edge_derivative(const T* u, const int8_t* solid, T wall, uint64_t i, int p, int n,
+                                             uint64_t st) {
+  if (n == 1) return T(0);
+  if (n == 2) {
+    const uint64_t i0 = i - uint64_t(p) * st;
+    return vel_at(u, solid, wall, i0 + st) - vel_at(u, solid, wall, i0);
+  }
+  if (p == 0)
+    return T(0.5) * (T(-3) * vel_at(u, solid, wall, i) + T(4) * vel_at(u, solid, wall, i + st) -
+                     vel_at(u, solid, wall, i + 2 * st));
+  if (p == n - 1)
+    return T(0.5) * (T(3) * vel_at(u, solid, wall, i) - T(4) * vel_at(u, solid, wall, i - st) +
+                     vel_at(u, solid, wall, i - 2 * st));
+  return T(0.5) * (vel_at(u, solid, wall, i + st) - vel_at(u, solid, wall, i - st));
+}
+
+// tau_eff = clip(tau + 3 nu_t, 0.5001, max(1, tau + 0.5)) (turbulence.py:875-888) with nu_t of
+// WALE (turbulence.py:754-841) or Vreman (turbulence.py:844-872), lattice filter width 1
+template <typename T>
+__global__ __launch_bounds__(256) void k_gradient_tau(const T* ux, const T* uy, const T* uz, const int8_t* solid,
+                                                      T wx, T wy, T wz, int nz, int ny, int nx, int model, T coef,
+                                                      T tau, T tau_cap, T* tau_out) {
+  const uint64_t n = uint64_t(nz) * ny * nx;
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = int(i % uint64_t(nx));
+  const int y = int((i / uint64_t(nx)) % uint64_t(ny));
+  const int z = int(i / (uint64_t(nx) * ny));
+  const T* u[3] = {ux, uy, uz};
+  const T w[3] = {wx, wy, wz};
+  const int pos[3] = {x, y, z};
+  const int len[3] = {nx, ny, nz};
+  const uint64_t st[3] = {1, uint64_t(nx), uint64_t(nx) * ny};
+  T g[3][3];  // g[i][j] = du_i / dx_j
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[c][d] = edge_derivative(u[c], solid, w[c], i, pos[d], len[d], st[d]);
+  T nu;
+  if (model == TLBM_SGS_WALE) {
+    T g2[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g2[r][c] = g[r][0] * g[0][c] + g[r][1] * g[1][c] + g[r][2] * g[2][c];
+    const T tr = (g2[0][0] + g2[1][1] + g2[2][2]) / T(3);
+    const T d01 = T(0.5) * (g2[0][1] + g2[1][0]), d02 = T(0.5) * (g2[0][2] + g2[2][0]),
+            d12 = T(0.5) * (g2[1][2] + g2[2][1]);
+    T sd = (g2[0][0] - tr) * (g2[0][0] - tr) + (g2[1][1] - tr) * (g2[1][1] - tr) + (g2[2][2] - tr) * (g2[2][2] - tr) +
+           T(2) * (d01 * d01 + d02 * d02 + d12 * d12);
+    const T s01 = T(0.5) * (g[0][1] + g[1][0]), s02 = T(0.5) * (g[0][2] + g[2][0]), s12 = T(0.5) * (g[1][2] + g[2][1]);
+    T sn = g[0][0] * g[0][0] + g[1][1] * g[1][1] + g[2][2] * g[2][2] + T(2) * (s01 * s01 + s02 * s02 + s12 * s12);
+    sd = sd > T(0) ? sd : T(0);
+    sn = sn > T(0) ? sn : T(0);
+    const T rsd = sqrt(sd), rsn = sqrt(sn);
+    nu = coef * coef * (sd * rsd) / (sn * sn * rsn + rsd * sqrt(rsd) + T(1e-30));
+  } else {
+    T b[3][3];  // beta = alpha^T alpha
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) b[r][c] = g[0][r] * g[0][c] + g[1][r] * g[1][c] + g[2][r] * g[2][c];
+    T a2 = T(0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a2 += g[r][c] * g[r][c];
+    T bb = b[0][0] * b[1][1] - b[0][1] * b[0][1] + b[0][0] * b[2][2] - b[0][2] * b[0][2] + b[1][1] * b[2][2] -
+           b[1][2] * b[1][2];
+    bb = bb > T(0) ? bb : T(0);
+    nu = coef * sqrt(bb / (a2 + T(1e-30)));
+  }
+  T te = tau + T(3) * nu;
+  te = te < T(0.5001) ? T(0.5001) : (te > tau_cap ? tau_cap : te);
+  tau_out[i] = te;
+}
+
+// ---------------------------------------------------------------------------
 // layout-aware FP64 reductions and scaling
 // ---------------------------------------------------------------------------
 
@@ -964,10 +1061,10 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
     for (int i = 0; i < 8; ++i) a.rate[i] = C(coll->rate[i]);
     a.tau_eff = coll->tau_eff;
     a.mlp = coll->mlp;
-    if (coll->tau_mode < TLBM_TAU_CONST || coll->tau_mode > TLBM_TAU_MLP)
+    if (coll->tau_mode < TLBM_TAU_CONST || coll->tau_mode > TLBM_TAU_FIELD64)
       return fail(TLBM_ERR_INVALID, "unknown tau_mode %d", coll->tau_mode);
-    if (coll->tau_mode == TLBM_TAU_FIELD && coll->tau_eff == nullptr)
-      return fail(TLBM_ERR_INVALID, "TLBM_TAU_FIELD needs tau_eff");
+    if ((coll->tau_mode == TLBM_TAU_FIELD || coll->tau_mode == TLBM_TAU_FIELD64) && coll->tau_eff == nullptr)
+      return fail(TLBM_ERR_INVALID, "a per-cell tau mode needs tau_eff");
     if (coll->tau_mode == TLBM_TAU_MLP) {
       if (coll->mlp == nullptr) return fail(TLBM_ERR_INVALID, "TLBM_TAU_MLP needs the packed closure (coll->mlp)");
       if (lattice != TLBM_D3Q19 || coll->kind != TLBM_COLL_BGK)
@@ -1203,6 +1300,7 @@ inline const char* step2_reason(int lattice, int store, const tlbm_collision* co
     return "two-step launches take float32 or 16-bit deviation storage";
   if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return "collision parameters required";
   if (coll->tau_mode == TLBM_TAU_MLP) return "the learned closure runs in the one-step kernel";
+  if (coll->tau_mode == TLBM_TAU_FIELD64) return "the fused step takes a float32 tau_eff field";
   if (bc != nullptr && (bc->kind == TLBM_BC_CHANNEL_X || bc->kind == TLBM_BC_FAR_FIELD)) {
     // the outlet copies x = nx-2, whose pulls reach x = nx-3: same tile as x = nx-1
     if (nx < 4 || (nx - 1) % tile2_width(lattice, store) < 2) return "the outlet column would straddle two tiles";
@@ -1461,6 +1559,8 @@ int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_stride
   if (rc) return rc;
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
   if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
+  if (coll->tau_mode == TLBM_TAU_FIELD64)
+    return fail(TLBM_ERR_UNSUPPORTED, "the fused step takes a float32 tau_eff field (TLBM_TAU_FIELD)");
   const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
   if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
   Call c{lattice, store, coll->kind, true, feat, 1, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
@@ -1669,6 +1769,36 @@ int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, 
       return check_launch("k_scale");
     });
   });
+}
+
+int tlbm_gradient_closure_tau(int dtype, const void* ux, const void* uy, const void* uz, const int8_t* solid,
+                              const double* wall_velocity, int nz, int ny, int nx, int model, double coef, double tau,
+                              void* tau_out, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (nz < 1 || ny < 1 || nx < 1) return fail(TLBM_ERR_SHAPE, "grid dims must be >= 1 (got %d,%d,%d)", nz, ny, nx);
+  const uint64_t n = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  if (n >= (uint64_t(1) << 31)) return fail(TLBM_ERR_SHAPE, "grid of %llu cells exceeds 2^31", (unsigned long long)n);
+  if (!ux || !uy || !uz || !tau_out) return fail(TLBM_ERR_INVALID, "null pointer");
+  if (model != TLBM_SGS_WALE && model != TLBM_SGS_VREMAN) return fail(TLBM_ERR_INVALID, "unknown closure %d", model);
+  if (!(coef >= 0.0)) return fail(TLBM_ERR_INVALID, "closure coefficient must be non-negative (got %g)", coef);
+  if (!(tau > 0.5)) return fail(TLBM_ERR_INVALID, "tau must be > 0.5 (got %g)", tau);
+  double w[3] = {0.0, 0.0, 0.0};
+  if (wall_velocity != nullptr)
+    for (int k = 0; k < 3; ++k) w[k] = wall_velocity[k];
+  const double cap = tau + 0.5 > 1.0 ? tau + 0.5 : 1.0;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == TLBM_STORE_F32) {
+    hipLaunchKernelGGL((k_gradient_tau<float>), dim3(blocks_for(n)), dim3(256), 0, s, (const float*)ux,
+                       (const float*)uy, (const float*)uz, solid, float(w[0]), float(w[1]), float(w[2]), nz, ny, nx,
+                       model, float(coef), float(tau), float(cap), (float*)tau_out);
+  } else if (dtype == TLBM_STORE_F64) {
+    hipLaunchKernelGGL((k_gradient_tau<double>), dim3(blocks_for(n)), dim3(256), 0, s, (const double*)ux,
+                       (const double*)uy, (const double*)uz, solid, w[0], w[1], w[2], nz, ny, nx, model, coef, tau,
+                       cap, (double*)tau_out);
+  } else {
+    return fail(TLBM_ERR_DTYPE, "the gradient closures take float32 or float64 velocity fields");
+  }
+  return check_launch("k_gradient_tau");
 }
 
 }  // extern "C"

@@ -23,6 +23,9 @@ the eager ops (the fused solver also takes 16-bit storage).
 
 from __future__ import annotations
 
+import ctypes
+import math
+
 import torch
 
 from . import _native as nat
@@ -38,7 +41,7 @@ __all__ = [
     "collide_smagorinsky_mrt3d", "stream",
     "stream3d", "stream3d_roll", "stream27", "stream27_roll", "correct_mass", "correct_mass3d",
     "correct_mass27", "bounce_back_cells", "bounce_back_cells_3d", "bounce_back_cells_27",
-    "compute_obstacle_forces_3d", "moment_sums",
+    "compute_obstacle_forces_3d", "moment_sums", "gradient_closure_tau",
 ]
 
 _EAGER = (torch.float32, torch.float64)
@@ -262,19 +265,66 @@ def collide_cumulant_d3q19(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C
     """D3Q19 Hermite-regularised cumulant (cumulant.py:1159-1377).
 
     ``C_s > 0`` applies that function's unclamped per-cell Smagorinsky τ_eff.
-    The gradient closures (WALE ``C_w``, Vreman ``C_v``) are not on this path.
+    ``C_w`` / ``C_v`` apply the WALE / Vreman closures: τ_eff from the velocity
+    gradient (``gradient_closure_tau``), with ``wall_velocity`` on ``solid_mask``.
     """
-    del solid_mask, wall_velocity
     if min(C_s, C_w, C_v) < 0.0:
         raise ValueError("SGS coefficients must be non-negative")
     if sum(v > 0.0 for v in (C_s, C_w, C_v)) > 1:
         raise ValueError("only one SGS model may be active")
+    if len(wall_velocity) != 3 or not all(math.isfinite(float(v)) for v in wall_velocity):
+        raise ValueError("wall_velocity must contain three finite values")
+    if solid_mask is not None and (tuple(solid_mask.shape) != tuple(f.shape[1:]) or solid_mask.dtype is not torch.bool
+                                   or solid_mask.device != f.device):
+        raise ValueError("solid_mask must be bool with the population spatial shape")
+    tau = _tau(tau, "collide_cumulant_d3q19")
+    rates = (omega_b, omega_odd, omega_even)
     if C_w > 0.0 or C_v > 0.0:
-        raise NotImplementedError("WALE / Vreman closures are not implemented on the HIP path")
+        te = gradient_closure_tau(f, tau, C_w=C_w, C_v=C_v, solid_mask=solid_mask, wall_velocity=wall_velocity)
+        mode = nat.TAU_FIELD64 if te.dtype == torch.float64 else nat.TAU_FIELD
+        coll = nat.make_collision(nat.COLL_CUMULANT, tau, tau_mode=mode, eq_rho_floor=False, rates=rates, tau_eff=te)
+        out = _collide(f, 19, coll, "collide_cumulant_d3q19")
+        del te
+        return out
     mode = nat.TAU_SMAG_RAW if C_s > 0.0 else nat.TAU_CONST
-    coll = nat.make_collision(nat.COLL_CUMULANT, _tau(tau, "collide_cumulant_d3q19"), tau_mode=mode,
-                              eq_rho_floor=False, cs_delta_sq=C_s * C_s, rates=(omega_b, omega_odd, omega_even))
+    coll = nat.make_collision(nat.COLL_CUMULANT, tau, tau_mode=mode, eq_rho_floor=False, cs_delta_sq=C_s * C_s,
+                              rates=rates)
     return _collide(f, 19, coll, "collide_cumulant_d3q19")
+
+
+def gradient_closure_tau(f, tau, *, C_w=0.0, C_v=0.0, solid_mask=None, wall_velocity=(0.0, 0.0, 0.0)):
+    """Per-cell τ_eff of the WALE (``C_w``) or Vreman (``C_v``) closure of a D3Q19 state.
+
+    τ_eff = clip(τ + 3 ν_t, 0.5001, max(1, τ + 0.5)) (turbulence.py:875-888), ν_t from the
+    macroscopic velocity with ``wall_velocity`` on ``solid_mask`` (cumulant.py:1263-1303),
+    differentiated centrally inside and one-sidedly on the faces (turbulence.py:595-695).
+    Returns a (nz, ny, nx) tensor in the populations' dtype (float32 or float64): the
+    ``tau_eff`` input of ``collide_*`` and of the fused step (``TLBM_TAU_FIELD``).
+    """
+    if (C_w > 0.0) == (C_v > 0.0):
+        raise ValueError("exactly one of C_w (WALE) and C_v (Vreman) must be positive")
+    if min(C_w, C_v) < 0.0:
+        raise ValueError("SGS coefficients must be non-negative")
+    fc, dims = _pops(f, 19, "gradient_closure_tau")
+    dt = _eager_dtype(fc, "gradient_closure_tau")
+    spatial = tuple(fc.shape[1:])
+    rho, ux, uy, uz = (torch.empty(spatial, dtype=fc.dtype, device=fc.device) for _ in range(4))
+    s = nat.stream_of(fc)
+    nat.check(nat.lib().tlbm_macroscopic(19, dt, fc.data_ptr(), rho.data_ptr(), ux.data_ptr(), uy.data_ptr(),
+                                         uz.data_ptr(), fc[0].numel(), s), "gradient_closure_tau")
+    mask = None
+    if solid_mask is not None:
+        if tuple(solid_mask.shape) != spatial:
+            raise ValueError("solid_mask must have the population spatial shape")
+        mask = solid_mask.to(device=fc.device, dtype=torch.int8).contiguous()
+    wall = (ctypes.c_double * 3)(*(float(v) for v in wall_velocity))
+    te = torch.empty(spatial, dtype=fc.dtype, device=fc.device)
+    model, coef = (nat.SGS_WALE, C_w) if C_w > 0.0 else (nat.SGS_VREMAN, C_v)
+    nat.check(nat.lib().tlbm_gradient_closure_tau(dt, ux.data_ptr(), uy.data_ptr(), uz.data_ptr(), nat.ptr(mask), wall,
+                                                  *dims, model, float(coef), float(tau), te.data_ptr(), s),
+              "gradient_closure_tau")
+    del rho
+    return te
 
 
 def collide_cumulant_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C_s=0.0):
