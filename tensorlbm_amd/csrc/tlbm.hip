@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256) void k_gradient_tau(const T* ux, const T* uy, 
     sd = sd > T(0) ? sd : T(0);
     sn = sn > T(0) ? sn : T(0);
     const T rsd = sqrt(sd), rsn = sqrt(sn);
-    nu = coef * coef * (sd * rsd) / (sn * sn * rsn + rsd * sqrt(rsd) + T(1e-30));
+    nu = coef * coef * (sd * rsd) / (sn * sn * rsn + sd * sqrt(rsd) + T(1e-30));
   } else {
     T b[3][3];  // beta = alpha^T alpha
 #pragma unroll
