@@ -143,7 +143,9 @@ int tlbm_step(int lattice, int store, const void* f_in, const int64_t* in_stride
  * run() loop, src/tensorlbm/triton_fused.py:892-895.  Same arguments and results as
  * tlbm_step(f_in -> tmp) then tlbm_step(tmp -> f_out), up to rounding.  D3Q19 /
  * D3Q27; F32, F16_DEV, BF16_DEV storage; every collision and boundary except
- * TAU_MLP.  force must be NULL: sample the Ladd force with tlbm_step.  Returns
+ * TAU_MLP.  force (D3Q19 only, needs solid): the Ladd force of BOTH steps is ADDED,
+ * the sum of what the two tlbm_step() launches add (the per-step force phase of
+ * triton_fused_obstacle.py:1494-1512, run every step).  Returns
  * TLBM_ERR_UNSUPPORTED for other arguments, which tlbm_step2_supported()
  * reports (force aside) without launching.
  */

@@ -500,7 +500,9 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   constexpr int Q = L::Q;
   constexpr int EX = TX + 2, EY = TY + 2, EC = EX * EY;
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
-  static_assert((FEAT & (FEAT_MLP | FEAT_FORCE)) == 0, "the closure and the force sample run in the one-step kernel");
+  constexpr bool kForce = (FEAT & FEAT_FORCE) != 0;
+  static_assert((FEAT & FEAT_MLP) == 0, "the closure runs in the one-step kernel");
+  static_assert(!kForce || !BoxWalls, "no force sample on the cavity walls");
   // step-one states, one row of EC tile cells per (set, slot, population): the
   // c_z = -1 / 0 / +1 populations of a plane are pulled while it is the newest /
   // middle / oldest of three, and each set has one slot more than it has planes
@@ -545,7 +547,14 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
   const int ux_ = min(x0 + (h2 ? t % TX : 0), nx - 1), uy_ = min(y0 + (h2 ? t / TX : 0), ny - 1);
   const int ox = ux_ - x0, oy = uy_ - y0;
   const bool own = h2 && x0 + t % TX < nx && y0 + t / TX < ny;
-  double unused_force = 0.0;  // no force sample here: tlbm_step2 takes no force buffer
+  double unused_force = 0.0;
+  // Ladd force of both steps (obstacles.py:91-130), each cell of the launch's output range
+  // sampled once per step: step one by the lanes whose halo-tile cell is an interior cell of
+  // this column tile inside the grid (no wrapped copy) on the chunk's own planes, step two by
+  // the lanes that own their output cell.  One FP64 atomic per wave after the march.
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  bool sampled = false;
+  const bool own1 = h1 && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && gx0 < nx && gy0 < ny;
 
   if constexpr (kAll) {
     if (t == 0) {
@@ -588,8 +597,20 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     for (int q = 0; q < Q; ++q) f[q] = pre[q];
     auto unused = [](int, int, int, int) { return C(0); };
     const bool solid = pre_mask != 0;
-    update_cell<L, S, COLL, true, FEAT, BCN_IN_F, BCK>(f, a, gx, gy, pz, cell, solid, nb_edge, unused_force,
-                                                       unused_force, unused_force, unused, eq_unit);
+    if constexpr (kForce) {
+      double sx = 0.0, sy = 0.0, sz = 0.0;
+      update_cell<L, S, COLL, true, FEAT, BCN_IN_F, BCK>(f, a, gx, gy, pz, cell, solid, nb_edge, sx, sy, sz, unused,
+                                                         eq_unit);
+      if (own1 && solid && p >= z0 && p < zend) {
+        fx += sx;
+        fy += sy;
+        fz += sz;
+        sampled = true;
+      }
+    } else {
+      update_cell<L, S, COLL, true, FEAT, BCN_IN_F, BCK>(f, a, gx, gy, pz, cell, solid, nb_edge, unused_force,
+                                                         unused_force, unused_force, unused, eq_unit);
+    }
     const int row_a = set_row(-1, j), row_b = set_row(0, j), row_c = set_row(1, j);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -640,8 +661,20 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
       const uint32_t cell = (uint32_t(z) * uint32_t(ny) + uint32_t(uy_)) * uint32_t(nx) + uint32_t(ux_);
       bool solid = false;
       if constexpr (kAll) solid = s_solid[((k + 1) % 3) * EC + (oy + 1) * EX + (ox + 1)] != 0;
-      update_cell<L, S, COLL, true, FEAT, BCN_PULL, BCK>(f, a, ux_, uy_, z, cell, solid, false, unused_force,
-                                                         unused_force, unused_force, lds_at, eq_unit);
+      if constexpr (kForce) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        update_cell<L, S, COLL, true, FEAT, BCN_PULL, BCK>(f, a, ux_, uy_, z, cell, solid, false, sx, sy, sz, lds_at,
+                                                           eq_unit);
+        if (own && solid) {
+          fx += sx;
+          fy += sy;
+          fz += sz;
+          sampled = true;
+        }
+      } else {
+        update_cell<L, S, COLL, true, FEAT, BCN_PULL, BCK>(f, a, ux_, uy_, z, cell, solid, false, unused_force,
+                                                           unused_force, unused_force, lds_at, eq_unit);
+      }
       T* fout = static_cast<T*>(a.fout);
       const uint32_t off = uint32_t(uy_) * uint32_t(a.osy) + uint32_t(ux_);
       // cavity rows (pitch = edge) rarely start on a cache line, and non-temporal
@@ -659,6 +692,9 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     } else {
       wait_vm_loads();
     }
+  }
+  if constexpr (kForce) {
+    if (a.force != nullptr) force_flush(a.force, fx, fy, fz, sampled);
   }
 }
 
@@ -1387,6 +1423,18 @@ int run_coll(const Call& c) {
     }
     if (c.steps == 2) {
       if constexpr (step2_store<L, S>() && COLL != TLBM_COLL_NONE) {
+        if (c.force != nullptr) {
+          // the Ladd force sample (a D3Q19 obstacle quantity, triton_fused_obstacle.py:1494-1512):
+          // the channel gets its own BGK kernel, the rest read the boundary kind at run time
+          if constexpr (L::Q == 19) {
+            if constexpr (COLL == TLBM_COLL_BGK) {
+              if (a.bc_kind == TLBM_BC_CHANNEL_X) return launch_lbm2<L, S, COLL, FEAT_ALL, TLBM_BC_CHANNEL_X>(a, c.s);
+            }
+            return launch_lbm2<L, S, COLL, FEAT_ALL, BCK_RUNTIME>(a, c.s);
+          } else {
+            return fail(TLBM_ERR_UNSUPPORTED, "the two-step force sample is built for D3Q19");
+          }
+        }
         if (c.feat) return launch_lbm2_bc<L, S, COLL>(a, c.s);
         return launch_lbm2<L, S, COLL, FEAT_NONE, TLBM_BC_NONE>(a, c.s);
       } else {
@@ -1578,10 +1626,11 @@ int tlbm_step2(int lattice, int store, const void* f_in, const int64_t* in_strid
   if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; a pull step needs two lattices");
   if (coll == nullptr) return fail(TLBM_ERR_INVALID, "collision parameters required");
   if (const char* why = step2_reason(lattice, store, coll, bc, nx)) return fail(TLBM_ERR_UNSUPPORTED, "%s", why);
-  if (force != nullptr)
-    return fail(TLBM_ERR_UNSUPPORTED, "the two-step launch takes no force buffer; sample the force with tlbm_step");
-  const bool feat = solid != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
-  Call c{lattice, store, coll->kind, true, feat, 2, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, nullptr,
+  if (force != nullptr && solid == nullptr) return fail(TLBM_ERR_INVALID, "force needs a solid mask");
+  if (force != nullptr && lattice != TLBM_D3Q19)
+    return fail(TLBM_ERR_UNSUPPORTED, "the two-step force sample is built for D3Q19; sample D3Q27 with tlbm_step");
+  const bool feat = solid != nullptr || force != nullptr || (bc != nullptr && bc->kind != TLBM_BC_NONE);
+  Call c{lattice, store, coll->kind, true, feat, 2, f_in, is, f_out, os, nz, ny, nx, coll, solid, bc, force,
          (hipStream_t)stream};
   return run(c);
 }

@@ -357,9 +357,9 @@ class FusedSolver3D:
                             deviation=self.deviation, what="FusedSolver3D.step")
 
     def step2(self, f: torch.Tensor, *, force: torch.Tensor | None = None) -> torch.Tensor:
-        """Advance two steps (one launch when ``two_step``); ``force`` receives both steps' samples
-        (the Ladd force is sampled by the one-step kernel, so passing it takes two launches)."""
-        if not self._two_step or force is not None:
+        """Advance two steps (one launch when ``two_step``); ``force`` receives both steps' samples.
+        D3Q19 samples the Ladd force inside the two-step launch; D3Q27 takes two one-step launches."""
+        if not self._two_step or (force is not None and self._Q != 19):
             return self.step(self.step(f, force=force), force=force)
         self._check_input(f)
         out = self._target(f, force)
