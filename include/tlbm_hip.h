@@ -6,9 +6,11 @@
  *   - Populations are SoA ``f[Q][nz][ny][nx]``, x fastest, contiguous (the
  *     reference layout, src/tensorlbm/d3q19.py:95-103).  D2Q9 fields
  *     ``f[9][ny][nx]`` are passed with nz = 1.
- *   - All pointers are device pointers owned by the caller; the library owns
- *     no device memory.  Work is enqueued on ``stream`` (a hipStream_t; 0 is
- *     the null stream) and nothing synchronises.
+ *   - All pointers are device pointers owned by the caller; the library
+ *     allocates no device memory.  Its one device array of its own is the
+ *     2.76 MB static workspace of tlbm_mlp_train_step_noalloc.  Work is
+ *     enqueued on ``stream`` (a hipStream_t; 0 is the null stream) and nothing
+ *     synchronises.
  *   - Streaming entry points require ``f_in != f_out``: a pull step reads
  *     neighbouring cells, so aliased buffers race across workgroups
  *     (reference src/tensorlbm/triton_fused.py:697-701).
@@ -245,6 +247,22 @@ int tlbm_mlp_grad(const float* params, const float* x, const float* t, int64_t n
 int tlbm_mlp_adam(float* params, const float* partials, int nblocks, float* m, float* v, int step,
                   float lr, float beta1, float beta2, float eps, float clip_norm, float* loss_out,
                   float* grad_out, tlbm_stream_t stream);
+
+/*
+ * tlbm_mlp_grad + tlbm_mlp_adam in one call that allocates no device memory: same arguments as
+ * those two (nblocks is derived from n), same kernels, same results bit for bit.  No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc).  The gradient partials live in a library-owned
+ * device array of TLBM_MLP_NOALLOC_MAX_SAMPLES / batch_block blocks (2.76 MB, one copy per GPU,
+ * held until the process ends); n above TLBM_MLP_NOALLOC_MAX_SAMPLES returns TLBM_ERR_SHAPE
+ * without launching.  Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (several streams or host threads) race and must be serialised by the caller.
+ */
+#define TLBM_MLP_NOALLOC_MAX_SAMPLES 32768
+int tlbm_mlp_train_step_noalloc(float* params, const float* x, const float* t, int64_t n, float* m,
+                                float* v, int step, float lr, float beta1, float beta2, float eps,
+                                float clip_norm, float* loss_out, float* grad_out,
+                                tlbm_stream_t stream);
 
 /* block average by an integer factor (_coarsen_mean, ai/pipeline.py:93-107, in 3-D):
  * out[10][nz/f][ny/f][nx/f] = rho, u_x, u_y, u_z, <uxux>, <uyuy>, <uzuz>, <uxuy>, <uxuz>, <uyuz> */

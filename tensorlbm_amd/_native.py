@@ -59,6 +59,7 @@ class Cavity(ctypes.Structure):
 
 
 CAVITIES_MAX = 128
+MLP_NOALLOC_MAX_SAMPLES = 32768  # TLBM_MLP_NOALLOC_MAX_SAMPLES
 
 
 class Boundary(ctypes.Structure):
@@ -120,6 +121,8 @@ _SIGNATURES = {
     "tlbm_mlp_grad": ([_VP, _VP, _VP, _I64, _VP, _VP], _I),
     "tlbm_mlp_adam": ([_VP, _VP, _I, _VP, _VP, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                        ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
+    "tlbm_mlp_train_step_noalloc": ([_VP, _VP, _VP, _I64, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
     "tlbm_coarsen_moments": ([_I, _I, _VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "tlbm_sgs_samples": ([_VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_cavities_layout": ([_VP, _I, _VP, _VP], _I),

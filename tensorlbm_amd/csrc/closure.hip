@@ -15,6 +15,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "../../include/tlbm_hip.h"
 #include "collide.hpp"
 #include "mlp.hpp"
@@ -222,6 +224,14 @@ __global__ __launch_bounds__(ADAM_THREADS) void k_mlp_adam(float* __restrict__ p
   }
 }
 
+// Gradient partials of tlbm_mlp_train_step_noalloc: up to NOALLOC_MAX_BLOCKS workgroups of
+// GRAD_BS samples, (NPARAM + 1) floats each (2.76 MB), one copy per GPU for the life of the
+// process.  k_mlp_grad writes every entry of the blocks a call launches before k_mlp_adam
+// reads them, so nothing is cleared between calls.
+constexpr int NOALLOC_MAX_BLOCKS = TLBM_MLP_NOALLOC_MAX_SAMPLES / GRAD_BS;
+static_assert(NOALLOC_MAX_BLOCKS * GRAD_BS == TLBM_MLP_NOALLOC_MAX_SAMPLES, "whole blocks");
+static __device__ __align__(256) float scratch[NOALLOC_MAX_BLOCKS * (NPARAM + 1)];
+
 // ---------------------------------------------------------------------------
 // DNS -> LES data
 // ---------------------------------------------------------------------------
@@ -328,6 +338,33 @@ int claunch(const char* what) {
   return TLBM_OK;
 }
 
+// Device address of `scratch` on the current device.  Host code must never pass the array's
+// name to a kernel: on the host it is a stand-in, not the device copy's address.  Looked up once
+// per device (each GPU holds its own copy at its own address) and kept.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[MAX_DEVICES] = {};
+
+int scratch_ptr(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return cfail(TLBM_ERR_HIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= MAX_DEVICES) return cfail(TLBM_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(tlbm::scratch));
+    if (e != hipSuccess || p == nullptr) {
+      char buf[256];
+      snprintf(buf, sizeof(buf), "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+      return tlbm::set_error(TLBM_ERR_HIP, buf);
+    }
+    g_scratch[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch[dev];
+  return TLBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -375,6 +412,34 @@ int tlbm_mlp_adam(float* params, const float* partials, int nblocks, float* m, f
   const float bc2 = 1.0f - powf(beta2, float(step));
   hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(ADAM_THREADS), 0, (hipStream_t)stream, params, partials, nblocks, m,
                      v, lr, beta1, beta2, eps, bc1, bc2, clip_norm, loss_out, grad_out);
+  return claunch("k_mlp_adam");
+}
+
+int tlbm_mlp_train_step_noalloc(float* params, const float* x, const float* t, int64_t n, float* m, float* v,
+                                int step, float lr, float beta1, float beta2, float eps, float clip_norm,
+                                float* loss_out, float* grad_out, tlbm_stream_t stream) {
+  if (n <= 0) return cfail(TLBM_ERR_SHAPE, "need at least one sample");
+  if (n > TLBM_MLP_NOALLOC_MAX_SAMPLES) {
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%lld samples exceed the no-allocation limit of %d per call", (long long)n,
+             TLBM_MLP_NOALLOC_MAX_SAMPLES);
+    return cfail(TLBM_ERR_SHAPE, buf);
+  }
+  if (step < 1) return cfail(TLBM_ERR_INVALID, "need step >= 1");
+  if (!params || !x || !t || (lr != 0.0f && (!m || !v))) return cfail(TLBM_ERR_INVALID, "null pointer");
+  float* partials = nullptr;
+  const int rc = scratch_ptr(&partials);
+  if (rc != TLBM_OK) return rc;
+  const int nb = int((n + GRAD_BS - 1) / GRAD_BS);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(uint32_t(nb)), dim3(256), 0, s, params, x, t, uint64_t(n), 1.0f / float(n),
+                     partials);
+  const int rg = claunch("k_mlp_grad");
+  if (rg != TLBM_OK) return rg;
+  const float bc1 = 1.0f - powf(beta1, float(step));
+  const float bc2 = 1.0f - powf(beta2, float(step));
+  hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(ADAM_THREADS), 0, s, params, partials, nb, m, v, lr, beta1, beta2,
+                     eps, bc1, bc2, clip_norm, loss_out, grad_out);
   return claunch("k_mlp_adam");
 }
 

@@ -115,3 +115,13 @@ def test_mlp_sizes_without_a_device():
     v = [ctypes.c_int64() for _ in range(4)]
     assert nat.lib().tlbm_mlp_sizes(*[ctypes.byref(x) for x in v]) == nat.OK
     assert [x.value for x in v] == [1345, 1359, 3520, 64]
+
+
+def test_noalloc_training_step_rejects_oversized_batches_before_any_launch():
+    lib = nat.lib()
+    p = ctypes.c_void_p(0x1000)
+    args = (p, p, 1, 1e-3, 0.9, 0.999, 1e-8, 1.0, None, None, None)
+    assert lib.tlbm_mlp_train_step_noalloc(p, p, p, nat.MLP_NOALLOC_MAX_SAMPLES + 1, *args) == nat.ERR_SHAPE
+    assert b"no-allocation limit" in lib.tlbm_last_error()
+    assert lib.tlbm_mlp_train_step_noalloc(p, p, p, 0, *args) == nat.ERR_SHAPE
+    assert lib.tlbm_mlp_train_step_noalloc(None, p, p, 64, *args) == nat.ERR_INVALID
