@@ -1,4 +1,5 @@
-"""Compile ``libtlbm_hip.so`` for gfx950 with hipcc (in-tree, no JIT cache)."""
+"""Compile ``libtlbm_hip.so`` for gfx950 with hipcc and the ``torch.ops.tlbm`` library over it
+(``libtlbm_torch.so``, host C++ against torch's headers); in-tree, no JIT cache."""
 
 from __future__ import annotations
 
@@ -34,6 +35,23 @@ def hipcc() -> str:
 
 SOURCES = ("tlbm.hip", "closure.hip", "cavities.hip")
 OBJ_DIR = PKG / "_build"
+TORCH_OPS_SRC = CSRC / "torch_ops.cpp"
+TORCH_OPS_OUT = PKG / "libtlbm_torch.so"
+
+
+def _torch_flags() -> tuple[list[str], list[str]]:
+    """Compile and link flags of the torch.ops.tlbm library (host C++ against torch's headers)."""
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cflags = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
+              f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}", f"-I{rocm}/include",
+              f"-I{ROOT / 'include'}", *(f"-I{p}" for p in ce.include_paths())]
+    tlib = ce.library_paths()[0]
+    lflags = ["-shared", f"-L{tlib}", f"-Wl,-rpath,{tlib}", f"-L{PKG}", "-Wl,-rpath,$ORIGIN", f"-L{rocm}/lib",
+              "-ltlbm_hip", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-lamdhip64"]
+    return cflags, lflags
 
 
 def build(verbose: bool = False) -> Path:
@@ -41,6 +59,12 @@ def build(verbose: bool = False) -> Path:
     OBJ_DIR.mkdir(exist_ok=True)
     common = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}"]
     jobs = []
+    tcflags, tlflags = _torch_flags()
+    tobj = OBJ_DIR / "torch_ops.o"
+    tcmd = ["g++", *tcflags, "-c", "-o", str(tobj), str(TORCH_OPS_SRC)]
+    if verbose:
+        print(" ".join(tcmd), flush=True)
+    tproc = subprocess.Popen(tcmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     for src in SOURCES:
         obj = OBJ_DIR / (Path(src).stem + ".o")
         cmd = [c for c in common if c != "-shared"] + ["-c", "-o", str(obj), str(CSRC / src)]
@@ -64,6 +88,19 @@ def build(verbose: bool = False) -> Path:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"link failed ({res.returncode})")
     os.replace(tmp, OUT)
+    out, err = tproc.communicate()
+    if tproc.returncode != 0:
+        sys.stderr.write(out + err)
+        raise RuntimeError("g++ failed for torch_ops.cpp")
+    ttmp = TORCH_OPS_OUT.with_suffix(".so.tmp")
+    tlink = ["g++", str(tobj), "-o", str(ttmp), *tlflags]
+    if verbose:
+        print(" ".join(tlink), flush=True)
+    res = subprocess.run(tlink, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"link of libtlbm_torch.so failed ({res.returncode})")
+    os.replace(ttmp, TORCH_OPS_OUT)
     return OUT
 
 
