@@ -32,12 +32,14 @@ from .ops import macroscopic3d
 __all__ = ["run_ai_les_pipeline_3d"]
 
 
-def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32, device="cuda:0"):
+def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32, device="cuda:0", keep_final=False):
     n = int(test[0].shape[1])
     f = cl.les_initial_populations(test[0])
     dev_store = dtype in (torch.float16, torch.bfloat16)
     if dev_store:
         f = (f - D3Q19.W_EXACT64.to(f.device, f.dtype).reshape(19, 1, 1, 1)).to(dtype)
+    else:
+        f = f.to(dtype)
     s = FusedSolver3D(n, n, n, tau, device, dtype, deviation=dev_store, closure=closure, C_s=C_s)
     errs, kes = [], []
     t0 = time.perf_counter()
@@ -52,15 +54,24 @@ def _les(test, tau, steps, every, *, closure=None, C_s=0.0, dtype=torch.float32,
         errs.append(float(torch.linalg.vector_norm(u - ref) / torch.linalg.vector_norm(ref)))
         kes.append(float((u * u).sum() / (ref * ref).sum()) - 1.0)
     torch.cuda.synchronize()
-    return {"err_final": errs[-1], "err_mean": sum(errs) / len(errs), "ke_err_final": kes[-1],
-            "ke_err_mean_abs": sum(abs(v) for v in kes) / len(kes), "wall_s": time.perf_counter() - t0}
+    out = {"err_final": errs[-1], "err_mean": sum(errs) / len(errs), "ke_err_final": kes[-1],
+           "ke_err_mean_abs": sum(abs(v) for v in kes) / len(kes), "wall_s": time.perf_counter() - t0}
+    if keep_final:
+        out["u_final"] = u
+    return out
 
 
 def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float = 0.51, steps: int = 300,
                            sample_every: int = 10, train_seed: int = 101, test_seed: int = 202, epochs: int = 30,
                            C_s: float = 0.17, les_dtype: torch.dtype = torch.float32,
                            device: str | torch.device = "cuda:0", log=None, k0: float = 4.0,
-                           u_rms: float = 0.03, max_samples: int = 1 << 25) -> dict:
+                           u_rms: float = 0.03, max_samples: int = 1 << 25, precision_check: bool = False) -> dict:
+    """Train on one DNS, validate LES runs against another; see the module docstring.
+
+    ``precision_check`` also runs the learned-closure LES in fp64 storage and reports the
+    velocity rel-L2 of the ``les_dtype`` run against it after the last step
+    (``learned_vs_fp64_rel_l2``).
+    """
     def say(msg):
         if log is not None:
             log(f"[les_pipeline {time.perf_counter() - t0:7.1f} s] {msg}")
@@ -89,12 +100,23 @@ def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float =
     say("test DNS done")
     runs = {}
     for name, kw in (("no_closure", {}), ("smagorinsky", {"C_s": C_s}), ("learned", {"closure": model, "C_s": C_s})):
-        runs[name] = _les(test, tau, steps, sample_every, dtype=les_dtype, device=device, **kw)
-        say(f"LES {name}: {runs[name]}")
-    return {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
-            "k0": k0, "u_rms": u_rms,
-            "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
-            "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
+        keep = precision_check and name == "learned"
+        runs[name] = _les(test, tau, steps, sample_every, dtype=les_dtype, device=device, keep_final=keep, **kw)
+        say(f"LES {name}: { {k: v for k, v in runs[name].items() if k != 'u_final'} }")
+    res = {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
+           "k0": k0, "u_rms": u_rms,
+           "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
+           "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
+    if precision_check:
+        u_low = runs["learned"].pop("u_final")
+        ref64 = _les(test, tau, steps, sample_every, dtype=torch.float64, device=device, keep_final=True,
+                     closure=model, C_s=C_s)
+        u64 = ref64.pop("u_final")
+        res["learned_fp64"] = ref64
+        res["learned_vs_fp64_rel_l2"] = float(torch.linalg.vector_norm(u_low - u64) / torch.linalg.vector_norm(u64))
+        say(f"learned closure, {res['les_dtype']} vs fp64 storage: velocity rel-L2 {res['learned_vs_fp64_rel_l2']:.3e}")
+        del u_low, u64
+    return res
 
 
 def main():
@@ -108,10 +130,14 @@ def main():
     p.add_argument("--les-dtype", choices=("float32", "float16"), default="float32")
     p.add_argument("--k0", type=float, default=4.0, help="spectrum peak of the initial field (box wavenumbers)")
     p.add_argument("--u-rms", type=float, default=0.03)
+    p.add_argument("--max-samples", type=int, default=1 << 25)
+    p.add_argument("--precision-check", action="store_true",
+                   help="also run the learned-closure LES in fp64 storage and compare")
     a = p.parse_args()
     res = run_ai_les_pipeline_3d(a.n_les, a.dns_scale, a.tau_dns, a.steps, a.every, epochs=a.epochs,
                                  les_dtype=getattr(torch, a.les_dtype), log=lambda m: print(m, flush=True),
-                                 k0=a.k0, u_rms=a.u_rms)
+                                 k0=a.k0, u_rms=a.u_rms, max_samples=a.max_samples,
+                                 precision_check=a.precision_check)
     print(json.dumps(res), flush=True)
 
 
