@@ -292,7 +292,7 @@ def main():
     closure = None
 
     def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True, mlp=False, two_step=None,
-             spread=False):
+             spread=False, force=False):
         f = f0.to(dtype)
         if deviation:
             f = (f0 - tl.lattice.D3Q19.W_EXACT64.to(dev).view(19, 1, 1, 1)).to(dtype)
@@ -303,6 +303,10 @@ def main():
                                   temporal_blocking=two_step)
         per = 2 if solver.two_step else 1
         launch = solver.step2 if per == 2 else solver.step
+        acc = None
+        if force:  # Ladd force of every step added to one FP64 accumulator (the caller zeroes it once)
+            acc = torch.zeros(3, dtype=torch.float64, device=dev)
+            launch = (lambda g: solver.step2(g, force=acc)) if per == 2 else (lambda g: solver.step(g, force=acc))
         if not warmed:
             f, _ = warm_clocks(launch, f, 1.0, torch)
             warmed.append(True)
@@ -325,6 +329,8 @@ def main():
             res["ms_per_launch_each_min_median_max"] = [round(min(each), 5), round(_median(each), 5),
                                                        round(max(each), 5)]
         res["finite"] = bool(torch.isfinite(f[:, ::16, ::16, ::16].float()).all())
+        if acc is not None:
+            res["force_finite"] = bool(torch.isfinite(acc).all())
         del solver, f
         return res
 
@@ -333,6 +339,7 @@ def main():
     forms = {}
     if not args.no_forms:
         forms["bgk_fp32_channel_sphere_one_step_per_launch"] = form(torch.float32, two_step=False)
+        forms["bgk_fp32_channel_sphere_ladd_force"] = form(torch.float32, force=True)
         forms["bgk_fp32_periodic_nomask"] = form(torch.float32, boundary=False, mask=False)
         forms["mrt_fp32_channel_sphere"] = form(torch.float32, collision="MRT")
         forms["bgk_fp16dev_channel_sphere"] = form(torch.float16, deviation=True)

@@ -140,6 +140,9 @@ extern __shared__ __align__(16) unsigned char dyn_lds[];
 // 1/x by the hardware reciprocal (1 ulp) in fp32; exact division in fp64
 __device__ __forceinline__ float approx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double approx_rcp(double x) { return 1.0 / x; }
+// square root by the hardware instruction (1 ulp) in fp32; exact in fp64
+__device__ __forceinline__ float approx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double approx_sqrt(double x) { return sqrt(x); }
 
 // Equilibrium of the boundary velocity u (a launch constant) computed where it
 // is used.  The empty asm makes the compiler treat the velocity as produced at
@@ -272,8 +275,15 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       // Smagorinsky tau as reference time scale, S = -Pi / (2 rho cs^2 tau_ref)
       C pi[6];
       neq_stress_moments<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
-      const C te_ref = smagorinsky_tau(pi_norm(pi), mc.rho_safe, a.tau, a.cs_delta_sq, a.tau_cap);
-      // features feed an fp16 network: hardware reciprocal / square root (1 ulp)
+      // features feed an fp16 network: the clamped Smagorinsky time scale (smagorinsky_tau)
+      // and the strain scale take hardware reciprocals and square roots (1 ulp)
+      const C pn = approx_sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                               C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
+      C disc = fma(a.tau, a.tau, C(18) * a.cs_delta_sq * pn * approx_rcp(mc.rho_safe));
+      disc = disc > C(0) ? disc : C(0);
+      C te_ref = C(0.5) * (a.tau + approx_sqrt(disc));
+      te_ref = te_ref < C(0.5001) ? C(0.5001) : te_ref;
+      te_ref = te_ref > a.tau_cap ? a.tau_cap : te_ref;
       const C sc = C(-1.5) * approx_rcp(te_ref * mc.rho_safe);
       float xs[7];
 #pragma unroll
