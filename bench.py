@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--copy-probe", action="store_true", help="also time a plain device copy of the same bytes")
     p.add_argument("--skip", default="", help="comma-separated forms to leave out: cavities, dhit")
     p.add_argument("--one-step", action="store_true", help="headline form with one step per launch (k_lbm)")
+    p.add_argument("--layout", choices=("padded", "contiguous"), default="padded",
+                   help="population layout of the headline form's solver buffers")
     return p.parse_args()
 
 
@@ -292,7 +294,7 @@ def main():
     closure = None
 
     def form(dtype, collision="BGK", deviation=False, boundary=True, mask=True, mlp=False, two_step=None,
-             spread=False, force=False):
+             spread=False, force=False, layout="padded"):
         f = f0.to(dtype)
         if deviation:
             f = (f0 - tl.lattice.D3Q19.W_EXACT64.to(dev).view(19, 1, 1, 1)).to(dtype)
@@ -300,7 +302,7 @@ def main():
                                   deviation=deviation, solid=solid if mask else None,
                                   boundary="channel_x" if boundary else None, u_bc=(u_in, 0.0, 0.0),
                                   closure=closure if mlp else None, C_s=0.17 if mlp else 0.0,
-                                  temporal_blocking=two_step)
+                                  temporal_blocking=two_step, layout=layout)
         per = 2 if solver.two_step else 1
         launch = solver.step2 if per == 2 else solver.step
         acc = None
@@ -335,7 +337,7 @@ def main():
         return res
 
     warmed = []
-    head = form(torch.float32, two_step=False if args.one_step else None, spread=True)
+    head = form(torch.float32, two_step=False if args.one_step else None, spread=True, layout=args.layout)
     forms = {}
     if not args.no_forms:
         forms["bgk_fp32_channel_sphere_one_step_per_launch"] = form(torch.float32, two_step=False)
@@ -401,6 +403,7 @@ def main():
             "lattice": "D3Q19",
             "collision": "BGK",
             "storage": "fp32",
+            "layout": args.layout,
             "solid_cells": int(solid.sum().item()),
             "seed": SEED,
             "timing": f"median of {REPS} windows of {args.steps} steps after {args.warmup} warm-up steps and a "
