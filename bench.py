@@ -355,6 +355,9 @@ def main():
         closure.set_feature_stats(torch.zeros(7), torch.full((7,), 1e-3))
         forms["bgk_fp16dev_mlp_channel_sphere"] = form(torch.float16, deviation=True, mlp=True)
         forms["bgk_fp32_mlp_channel_sphere"] = form(torch.float32, mlp=True)
+        # the same network with bf16 matrix-core operands ("BF16 weights, FP32 accumulation")
+        closure = EddyViscosityMLP3D(dev, params=closure.params, operands="bf16")
+        forms["bgk_fp16dev_mlp_bf16_channel_sphere"] = form(torch.float16, deviation=True, mlp=True)
         # the closure form against the headline (fp32, two steps per launch) and against the plain
         # one-step fp32 kernel
         mlp_rate = forms["bgk_fp16dev_mlp_channel_sphere"]["mlups"]

@@ -69,6 +69,8 @@ typedef struct ihipStream_t* tlbm_stream_t; /* == hipStream_t */
                             nu_t from the tlbm_mlp_pack blob in coll->mlp (D3Q19 BGK) */
 #define TLBM_TAU_FIELD64 5 /* per-cell tau_eff array of float64 values (coll->tau_eff points to doubles);
                               tlbm_collide only -- the fp64 form of TLBM_TAU_FIELD */
+#define TLBM_TAU_MLP_BF16 6 /* TLBM_TAU_MLP with the bf16-operand network: coll->mlp from tlbm_mlp_pack_bf16
+                               (D3Q19 BGK; fp32, fp64, fp16- or bf16-deviation storage) */
 
 /* gradient eddy-viscosity closures for tlbm_gradient_closure_tau */
 #define TLBM_SGS_WALE 1   /* _wale_nu_t_3d, turbulence.py:754-841 */
@@ -232,6 +234,11 @@ int tlbm_mlp_sizes(int64_t* nparam, int64_t* ntotal, int64_t* packed_bytes, int6
 
 /* fp32 parameters -> packed fp16 matrix-core blob used by tlbm_step with TLBM_TAU_MLP (coll->mlp) */
 int tlbm_mlp_pack(const float* params, void* packed, tlbm_stream_t stream);
+
+/* the same blob with bf16 operands ("BF16 weights, FP32 accumulation"): W1 and W2 as bf16 values,
+ * the inputs, the layer-1 bias and the layer-1 activations as two-term bf16 splits; used by
+ * tlbm_step with TLBM_TAU_MLP_BF16.  Same size as the fp16 blob. */
+int tlbm_mlp_pack_bf16(const float* params, void* packed, tlbm_stream_t stream);
 
 /* y[n] = MLP(x[n][7]) in fp32 (predict_tau_eff_2d's network call, ai/inference.py:53-65) */
 int tlbm_mlp_predict(const float* params, const float* x, int64_t n, float* y, tlbm_stream_t stream);

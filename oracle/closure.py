@@ -105,6 +105,39 @@ def split_fp16(a):
     return hi, lo
 
 
+def bf16(a):
+    """Round fp32 values to bfloat16 (nearest, ties to even; finite inputs) and return as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def split_bf16(a):
+    """Two-term bf16 split a ≈ hi + lo, hi = bf16(a), lo = bf16(a − hi), of fp32 values."""
+    a32 = np.asarray(a, np.float32)
+    hi = bf16(a32)
+    lo = bf16((a32.astype(np.float64) - hi).astype(np.float32))
+    return hi, lo
+
+
+def forward_mfma_bf16(vec, x, mean, inv_std):
+    """ν_t of the bf16-operand form: bf16 W1 and W2; normalised inputs, the layer-1 bias and the
+    layer-1 activations relu(z1) as two-term bf16 splits; wide accumulation; fp32 output layer."""
+    p = unpack(np.asarray(vec, dtype=np.float64))
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
+    xh = f32((f32(x) - f32(mean)) * f32(inv_std))
+    xhi, xlo = split_bf16(xh)
+    w1 = bf16(p["w1"])
+    b1h, b1l = split_bf16(p["b1"])
+    z1 = f32(xhi @ w1.T + xlo @ w1.T + (b1h + b1l))
+    hh, hl = split_bf16(np.maximum(z1, 0.0))
+    w2 = bf16(p["w2"])
+    z2 = f32(hh @ w2.T + hl @ w2.T + f32(p["b2"]))
+    h2 = np.maximum(z2, 0.0)
+    z3 = h2 @ f32(p["w3"]) + f32(p["b3"])[0]
+    return softplus(z3)
+
+
 def forward_mfma(vec, x, mean, inv_std):
     """ν_t as the fused collision evaluates it on the matrix cores: fp16 W1 and W2, normalised
     inputs and the layer-1 bias as two-term fp16 splits, layer-1 activations relu(z1) rounded
@@ -240,12 +273,13 @@ def shell_spectrum(u):
 # ---------------------------------------------------------------------------
 
 
-def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_mfma=True):
+def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_mfma=True, operands="fp16"):
     """Periodic pull-stream then BGK with τ_eff = max(τ + 3 ν_t, 0.501) (ai/inference.py:56-92).
 
     ν_t = MLP(S, |S|) with S = −Π^neq / (2 ρ c_s² τ_ref) (turbulence.py:1449-1480), τ_ref the
     clamped Smagorinsky τ of ``cs_delta_sq`` (τ itself when it is 0).  ``params`` is the full
-    vector (trainable + mean + inv_std).  Returns (f_out, ν_t).
+    vector (trainable + mean + inv_std).  ``operands`` ("fp16" or "bf16") picks the matrix-core
+    form that ``emulate_mfma`` reproduces.  Returns (f_out, ν_t).
     """
     from . import ref
 
@@ -257,7 +291,8 @@ def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_mfma=True):
     pi = ref.neq_stress(fneq)
     te_ref = ref.smagorinsky_tau(tau, ref.pi_norm(fneq), rho_safe, cs_delta_sq)
     x = features_from_pi(pi, rho_safe, te_ref).reshape(-1, N_IN)
-    nu = forward_mfma(vec, x, mean, istd) if emulate_mfma else forward(vec, x, mean, istd)[0]
+    mfma = forward_mfma_bf16 if operands == "bf16" else forward_mfma
+    nu = mfma(vec, x, mean, istd) if emulate_mfma else forward(vec, x, mean, istd)[0]
     nu = nu.reshape(rho_safe.shape)
     te = np.maximum(tau + 3.0 * nu, 0.501)
     return fin - fneq / te[None], nu

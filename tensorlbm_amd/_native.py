@@ -29,7 +29,7 @@ ERR_HIP = -100
 D2Q9, D3Q19, D3Q27 = 9, 19, 27
 STORE_F32, STORE_F64, STORE_F16, STORE_BF16, STORE_F16_DEV, STORE_BF16_DEV = range(6)
 COLL_NONE, COLL_BGK, COLL_MRT, COLL_CM, COLL_CUMULANT, COLL_GEIER = range(6)
-TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP, TAU_FIELD64 = range(6)
+TAU_CONST, TAU_SMAG, TAU_FIELD, TAU_SMAG_RAW, TAU_MLP, TAU_FIELD64, TAU_MLP_BF16 = range(7)
 SGS_WALE, SGS_VREMAN = 1, 2
 BC_NONE, BC_CHANNEL_X, BC_LID_Y, BC_FAR_FIELD = range(4)
 
@@ -117,6 +117,7 @@ _SIGNATURES = {
     "tlbm_smagorinsky_tau_sum": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.c_double, ctypes.c_double, _VP, _VP], _I),
     "tlbm_mlp_sizes": ([_VP, _VP, _VP, _VP], _I),
     "tlbm_mlp_pack": ([_VP, _VP, _VP], _I),
+    "tlbm_mlp_pack_bf16": ([_VP, _VP, _VP], _I),
     "tlbm_mlp_predict": ([_VP, _VP, _I64, _VP, _VP], _I),
     "tlbm_mlp_grad": ([_VP, _VP, _VP, _I64, _VP, _VP], _I),
     "tlbm_mlp_adam": ([_VP, _VP, _I, _VP, _VP, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,

@@ -70,13 +70,21 @@ class EddyViscosityMLP3D:
 
     ``params`` layout: ``W1[32][7] b1[32] W2[32][32] b2[32] w3[32] b3 | mean[7] inv_std[7]``.
     ``model(x)`` returns ``(N, 1)`` ν_t ≥ 0 like the reference's ``EddyViscosityMLP.forward``.
+
+    ``operands`` picks the matrix-core form the fused collision evaluates: ``"fp16"`` (fp16
+    weights, inputs and layer-1 bias as two-term fp16 splits, fp16 activations) or ``"bf16"``
+    (bf16 weights, inputs, bias and layer-1 activations as two-term bf16 splits); both
+    accumulate in fp32.  Training and ``forward`` are fp32 either way.
     """
 
     in_features = N_IN
     hidden_features = N_HID
 
     def __init__(self, device: str | torch.device = "cuda:0", *, seed: int = 0,
-                 params: torch.Tensor | None = None) -> None:
+                 params: torch.Tensor | None = None, operands: str = "fp16") -> None:
+        if operands not in ("fp16", "bf16"):
+            raise ValueError(f"operands must be 'fp16' or 'bf16', got {operands!r}")
+        self.operands = operands
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError(f"EddyViscosityMLP3D lives on a GPU device, got {self.device}")
@@ -116,11 +124,18 @@ class EddyViscosityMLP3D:
         self.mark_updated()
 
     @property
+    def tau_mode(self) -> int:
+        """The fused collision's τ mode for this network's operand form."""
+        return nat.TAU_MLP_BF16 if self.operands == "bf16" else nat.TAU_MLP
+
+    @property
     def packed(self) -> torch.Tensor:
-        """Matrix-core operand blob (fp16 weights) for the fused collision, rebuilt when stale."""
+        """Matrix-core operand blob (fp16 or bf16 weights) for the fused collision, rebuilt when stale."""
         if self._packed_version != self._version:
-            nat.check(nat.lib().tlbm_mlp_pack(self.params.data_ptr(), self._packed.data_ptr(),
-                                              nat.stream_of(self.params)), "tlbm_mlp_pack")
+            lib = nat.lib()
+            pack = lib.tlbm_mlp_pack_bf16 if self.operands == "bf16" else lib.tlbm_mlp_pack
+            nat.check(pack(self.params.data_ptr(), self._packed.data_ptr(), nat.stream_of(self.params)),
+                      "tlbm_mlp_pack")
             self._packed_version = self._version
         return self._packed
 
@@ -374,4 +389,4 @@ def les_initial_populations(fields: torch.Tensor) -> torch.Tensor:
 
 
 def clone_model(model: EddyViscosityMLP3D) -> EddyViscosityMLP3D:
-    return EddyViscosityMLP3D(model.device, params=model.params)
+    return EddyViscosityMLP3D(model.device, params=model.params, operands=model.operands)

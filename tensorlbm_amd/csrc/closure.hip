@@ -35,18 +35,31 @@ constexpr int GRAD_BS = 64;  // samples per workgroup in the gradient kernel
 // fp16 bits of v, round to nearest even
 __device__ __forceinline__ uint16_t f16_bits(float v) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v)); }
 
+// bf16 bits of v, round to nearest even
+__device__ __forceinline__ uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
+
+// operand bits of v in the blob's type, and v rounded to that type
+template <bool BF16>
+__device__ __forceinline__ uint16_t op_bits(float v) {
+  return BF16 ? bf16_bits(v) : f16_bits(v);
+}
+template <bool BF16>
+__device__ __forceinline__ float op_round(float v) {
+  return BF16 ? static_cast<float>(static_cast<__bf16>(v)) : static_cast<float>(static_cast<_Float16>(v));
+}
+
+template <bool BF16>
 __global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ p, Packed* __restrict__ out) {
   const int l = threadIdx.x;
   const int h = l >> 5, i = l & 31;
   for (int e = 0; e < 8; ++e) {
     if (e < NIN) {
-      out->a1[l][e] = f16_bits(p[OW1 + i * NIN + e]);
+      out->a1[l][e] = op_bits<BF16>(p[OW1 + i * NIN + e]);
     } else {  // bias at K 7 (hi, lanes < 32) and K 15 (lo, lanes >= 32)
       const float b = p[OB1 + i];
-      const float bh = static_cast<float>(static_cast<_Float16>(b));
-      out->a1[l][e] = f16_bits(h == 0 ? b : b - bh);
+      out->a1[l][e] = op_bits<BF16>(h == 0 ? b : b - op_round<BF16>(b));
     }
-    for (int s = 0; s < 2; ++s) out->a2[s][l][e] = f16_bits(p[OW2 + i * NH + w2_col(s, h, e)]);
+    for (int s = 0; s < 2; ++s) out->a2[s][l][e] = op_bits<BF16>(p[OW2 + i * NH + w2_col(s, h, e)]);
   }
   if (l < 32) {
     const int hh = l >> 4, r = l & 15;
@@ -379,7 +392,13 @@ int tlbm_mlp_sizes(int64_t* nparam, int64_t* ntotal, int64_t* packed_bytes, int6
 
 int tlbm_mlp_pack(const float* params, void* packed, tlbm_stream_t stream) {
   if (!params || !packed) return cfail(TLBM_ERR_INVALID, "null pointer");
-  hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, (hipStream_t)stream, params, (mlp::Packed*)packed);
+  hipLaunchKernelGGL(k_mlp_pack<false>, dim3(1), dim3(64), 0, (hipStream_t)stream, params, (mlp::Packed*)packed);
+  return claunch("k_mlp_pack");
+}
+
+int tlbm_mlp_pack_bf16(const float* params, void* packed, tlbm_stream_t stream) {
+  if (!params || !packed) return cfail(TLBM_ERR_INVALID, "null pointer");
+  hipLaunchKernelGGL(k_mlp_pack<true>, dim3(1), dim3(64), 0, (hipStream_t)stream, params, (mlp::Packed*)packed);
   return claunch("k_mlp_pack");
 }
 

@@ -19,6 +19,14 @@
 // C layout.  The output layer is 16 local fp32 FMAs plus one cross-half
 // shuffle.  Six matrix-core tiles per wave.  The operand blob (3.5 KB) is read
 // straight from global memory, where it stays cache-resident across the launch.
+//
+// The bf16 form (nu_t_bf16, blob from tlbm_mlp_pack_bf16) runs the same tiles
+// with v_mfma_f32_32x32x16_bf16: W1 and W2 are single bf16 values (8 bits);
+// the normalised inputs and the layer-1 bias enter as two-term bf16 splits
+// (~16 bits) in the same K slots; the layer-1 activations become two B
+// operands, the bf16 value of relu(c) and the bf16 value of what it leaves out,
+// taken against the same W2 operand (four tiles per block for layer 2).  The
+// output layer is the same fp32 step.  Ten matrix-core tiles per wave.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -68,6 +76,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ f16x8 as_f16x8(u16x8 v) { return __builtin_bit_cast(f16x8, v); }
 
@@ -82,11 +91,13 @@ __device__ __forceinline__ float relu(float v) {
   return __builtin_bit_cast(float, __builtin_elementwise_max(__builtin_bit_cast(int, v), 0));
 }
 
-__device__ __forceinline__ f16x8 xor32(f16x8 v) {
+// the partner lane's (lane ^ 32) operand, for either 16-bit operand type
+template <typename V>
+__device__ __forceinline__ V xor32(V v) {
   u32x4 u = __builtin_bit_cast(u32x4, v);
 #pragma unroll
   for (int i = 0; i < 4; ++i) u[i] = uint32_t(__shfl_xor(int(u[i]), 32, 64));
-  return __builtin_bit_cast(f16x8, u);
+  return __builtin_bit_cast(V, u);
 }
 
 // relu(c[8s .. 8s+7]) rounded to fp16 (round to nearest even), as a layer-2 B operand
@@ -145,6 +156,66 @@ __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
   const float p0 = block_out(P, h, c, a20, a21);
   c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, zero, 0, 0, 0);
   const float p1 = block_out(P, h, c, a20, a21);
+  return softplus((h == 0 ? p0 : p1) + P->b3);
+}
+
+// ---- bf16 operands ----------------------------------------------------------
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u16x8 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// relu(c[8s .. 8s+7]) as a two-term bf16 split: hi = bf16(v), lo = bf16(v - hi), both
+// rounded to nearest even; hi + lo carries ~16 bits of v
+__device__ __forceinline__ void act_bf16(const f32x16& c, int s, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float v = relu(c[8 * s + e]);
+    const __bf16 h = static_cast<__bf16>(v);
+    hi[e] = h;
+    lo[e] = static_cast<__bf16>(v - static_cast<float>(h));
+  }
+}
+
+// layer 2 (hi and lo activations against the same W2 tile) + output for one 32-cell block
+__device__ __forceinline__ float block_out_bf16(const Packed* P, int h, const f32x16& c, bf16x8 a20, bf16x8 a21) {
+  f32x16 d;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) d[r] = P->b2c[h][r];
+  bf16x8 bh, bl;
+  act_bf16(c, 0, bh, bl);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, bh, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, bl, d, 0, 0, 0);
+  act_bf16(c, 1, bh, bl);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, bh, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, bl, d, 0, 0, 0);
+  float p = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) p += P->w3c[h][r] * relu(d[r]);
+  return p + __shfl_xor(p, 32, 64);
+}
+
+// nu_t of the bf16 blob (tlbm_mlp_pack_bf16); same calling rules as nu_t
+__device__ __forceinline__ float nu_t_bf16(const Packed* P, const float (&x)[7]) {
+  const int lane = __lane_id();
+  const int h = lane >> 5;
+  bf16x8 hi, lo;
+#pragma unroll
+  for (int e = 0; e < 7; ++e) {
+    const float v = (x[e] - P->mean[e]) * P->istd[e];
+    hi[e] = static_cast<__bf16>(v);
+    lo[e] = static_cast<__bf16>(v - static_cast<float>(hi[e]));
+  }
+  hi[7] = lo[7] = static_cast<__bf16>(1.0f);
+  const bf16x8 got = xor32(h == 0 ? lo : hi);
+  const bf16x8 b0 = h == 0 ? hi : got;
+  const bf16x8 b1 = h == 0 ? got : lo;
+  const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a1[lane]));
+  const bf16x8 a20 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
+  const bf16x8 a21 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
+  const f32x16 zero = {};
+  f32x16 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, zero, 0, 0, 0);
+  const float p0 = block_out_bf16(P, h, c, a20, a21);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, zero, 0, 0, 0);
+  const float p1 = block_out_bf16(P, h, c, a20, a21);
   return softplus((h == 0 ? p0 : p1) + P->b3);
 }
 

@@ -38,13 +38,15 @@
 namespace tlbm {
 
 // FEAT_BC: solid mask and boundaries; FEAT_FORCE: Ladd force sample; FEAT_MLP:
-// learned closure (whole waves stay alive for the matrix-core evaluation).
+// learned closure (whole waves stay alive for the matrix-core evaluation);
+// FEAT_MLP_BF16 with FEAT_MLP: its bf16-operand form.
 // Bits combine; the one-step kernel builds FEAT_ALL, the two-step kernel FEAT_BC.
 enum Feat {
   FEAT_NONE = 0,
   FEAT_BC = 1,
   FEAT_MLP = 2,
   FEAT_FORCE = 4,
+  FEAT_MLP_BF16 = 8,
   FEAT_ALL = FEAT_BC | FEAT_FORCE,
   FEAT_MLP_ALL = FEAT_MLP | FEAT_ALL
 };
@@ -291,7 +293,8 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       xs[6] = __builtin_amdgcn_sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
                                              2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
       // operands straight from the (cache-resident, 3.5 KB) blob: no staging barrier
-      const float nu = mlp::nu_t(static_cast<const mlp::Packed*>(a.mlp), xs);
+      const mlp::Packed* blob = static_cast<const mlp::Packed*>(a.mlp);
+      const float nu = (FEAT & FEAT_MLP_BF16) != 0 ? mlp::nu_t_bf16(blob, xs) : mlp::nu_t(blob, xs);
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
@@ -1107,12 +1110,12 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
     for (int i = 0; i < 8; ++i) a.rate[i] = C(coll->rate[i]);
     a.tau_eff = coll->tau_eff;
     a.mlp = coll->mlp;
-    if (coll->tau_mode < TLBM_TAU_CONST || coll->tau_mode > TLBM_TAU_FIELD64)
+    if (coll->tau_mode < TLBM_TAU_CONST || coll->tau_mode > TLBM_TAU_MLP_BF16)
       return fail(TLBM_ERR_INVALID, "unknown tau_mode %d", coll->tau_mode);
     if ((coll->tau_mode == TLBM_TAU_FIELD || coll->tau_mode == TLBM_TAU_FIELD64) && coll->tau_eff == nullptr)
       return fail(TLBM_ERR_INVALID, "a per-cell tau mode needs tau_eff");
-    if (coll->tau_mode == TLBM_TAU_MLP) {
-      if (coll->mlp == nullptr) return fail(TLBM_ERR_INVALID, "TLBM_TAU_MLP needs the packed closure (coll->mlp)");
+    if (coll->tau_mode == TLBM_TAU_MLP || coll->tau_mode == TLBM_TAU_MLP_BF16) {
+      if (coll->mlp == nullptr) return fail(TLBM_ERR_INVALID, "the learned closure needs its packed blob (coll->mlp)");
       if (lattice != TLBM_D3Q19 || coll->kind != TLBM_COLL_BGK)
         return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
     }
@@ -1345,7 +1348,8 @@ inline const char* step2_reason(int lattice, int store, const tlbm_collision* co
   if (store != TLBM_STORE_F32 && store != TLBM_STORE_F16_DEV && store != TLBM_STORE_BF16_DEV)
     return "two-step launches take float32 or 16-bit deviation storage";
   if (coll == nullptr || coll->kind == TLBM_COLL_NONE) return "collision parameters required";
-  if (coll->tau_mode == TLBM_TAU_MLP) return "the learned closure runs in the one-step kernel";
+  if (coll->tau_mode == TLBM_TAU_MLP || coll->tau_mode == TLBM_TAU_MLP_BF16)
+    return "the learned closure runs in the one-step kernel";
   if (coll->tau_mode == TLBM_TAU_FIELD64) return "the fused step takes a float32 tau_eff field";
   if (bc != nullptr && (bc->kind == TLBM_BC_CHANNEL_X || bc->kind == TLBM_BC_FAR_FIELD)) {
     // the outlet copies x = nx-2, whose pulls reach x = nx-3: same tile as x = nx-1
@@ -1429,6 +1433,20 @@ int run_coll(const Call& c) {
         return launch_lbm<L, S, COLL, true, FEAT_MLP>(a, bx, by, nb, c.s);
       } else {
         return fail(TLBM_ERR_UNSUPPORTED, "the learned closure is built for D3Q19 BGK");
+      }
+    }
+    if (c.coll != nullptr && c.coll->tau_mode == TLBM_TAU_MLP_BF16) {
+      // bf16-operand closure: fp32, fp64 and 16-bit deviation storage
+      if constexpr (L::Q == 19 && COLL == TLBM_COLL_BGK &&
+                    (S == ST_F32 || S == ST_F64 || S == ST_F16_DEV || S == ST_BF16_DEV)) {
+        constexpr int M = FEAT_MLP | FEAT_MLP_BF16;
+        if (!c.stream_) return fail(TLBM_ERR_UNSUPPORTED, "the learned closure runs in the fused step");
+        if (c.feat && c.force == nullptr) return launch_lbm_kind<L, S, COLL, M | FEAT_BC>(a, bx, by, nb, c.s);
+        if (c.feat) return launch_lbm<L, S, COLL, true, M | FEAT_ALL>(a, bx, by, nb, c.s);
+        return launch_lbm<L, S, COLL, true, M>(a, bx, by, nb, c.s);
+      } else {
+        return fail(TLBM_ERR_UNSUPPORTED,
+                    "the bf16 closure is built for D3Q19 BGK with fp32, fp64 or 16-bit deviation storage");
       }
     }
     if (c.steps == 2) {

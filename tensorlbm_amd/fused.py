@@ -297,7 +297,7 @@ class FusedSolver3D:
                 raise NotImplementedError("the learned closure runs in the D3Q19 BGK kernel")
             if not hasattr(closure, "packed"):
                 raise TypeError("closure must be an EddyViscosityMLP3D")
-            mode = nat.TAU_MLP
+            mode = getattr(closure, "tau_mode", nat.TAU_MLP)
         self._coll = nat.make_collision(COLLISIONS[col], self.tau, tau_mode=mode,
                                         cs_delta_sq=C_s * C_s * delta * delta, rates=r)
         self.solid = None

@@ -65,12 +65,14 @@ def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float =
                            sample_every: int = 10, train_seed: int = 101, test_seed: int = 202, epochs: int = 30,
                            C_s: float = 0.17, les_dtype: torch.dtype = torch.float32,
                            device: str | torch.device = "cuda:0", log=None, k0: float = 4.0,
-                           u_rms: float = 0.03, max_samples: int = 1 << 25, precision_check: bool = False) -> dict:
+                           u_rms: float = 0.03, max_samples: int = 1 << 25, precision_check: bool = False,
+                           closure_operands: str = "fp16") -> dict:
     """Train on one DNS, validate LES runs against another; see the module docstring.
 
     ``precision_check`` also runs the learned-closure LES in fp64 storage and reports the
     velocity rel-L2 of the ``les_dtype`` run against it after the last step
-    (``learned_vs_fp64_rel_l2``).
+    (``learned_vs_fp64_rel_l2``).  ``closure_operands`` ("fp16" or "bf16") picks the matrix-core
+    form the fused collision evaluates the trained network in.
     """
     def say(msg):
         if log is not None:
@@ -93,6 +95,8 @@ def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float =
     model, meta = cl.train_eddy_viscosity_model_3d(feats, labels, cl.TrainConfig3D(epochs=epochs), device=device,
                                                    log=say)
     del feats, labels
+    if closure_operands != "fp16":
+        model = cl.EddyViscosityMLP3D(model.device, params=model.params, operands=closure_operands)
     say(f"trained: val R2 {meta['final_val_r2']:.3f} in {meta['training_time_s']:.1f} s")
     test = cl.run_dns_reference_3d(n_les, tau_dns, steps, sample_every, test_seed, device, dns_scale=dns_scale,
                                    k0=k0, u_rms=u_rms)
@@ -105,7 +109,8 @@ def run_ai_les_pipeline_3d(n_les: int = 32, dns_scale: int = 3, tau_dns: float =
         say(f"LES {name}: { {k: v for k, v in runs[name].items() if k != 'u_final'} }")
     res = {"n_les": n_les, "n_dns": n_les * dns_scale, "tau_dns": tau_dns, "tau_les": tau, "steps": steps,
            "k0": k0, "u_rms": u_rms,
-           "les_dtype": str(les_dtype).replace("torch.", ""), "samples_train": meta["n_samples_train"],
+           "les_dtype": str(les_dtype).replace("torch.", ""), "closure_operands": closure_operands,
+           "samples_train": meta["n_samples_train"],
            "val_r2": meta["final_val_r2"], "train_s": meta["training_time_s"], "dns_s": t_dns, "les": runs}
     if precision_check:
         u_low = runs["learned"].pop("u_final")
@@ -131,13 +136,14 @@ def main():
     p.add_argument("--k0", type=float, default=4.0, help="spectrum peak of the initial field (box wavenumbers)")
     p.add_argument("--u-rms", type=float, default=0.03)
     p.add_argument("--max-samples", type=int, default=1 << 25)
+    p.add_argument("--closure-operands", choices=("fp16", "bf16"), default="fp16")
     p.add_argument("--precision-check", action="store_true",
                    help="also run the learned-closure LES in fp64 storage and compare")
     a = p.parse_args()
     res = run_ai_les_pipeline_3d(a.n_les, a.dns_scale, a.tau_dns, a.steps, a.every, epochs=a.epochs,
                                  les_dtype=getattr(torch, a.les_dtype), log=lambda m: print(m, flush=True),
                                  k0=a.k0, u_rms=a.u_rms, max_samples=a.max_samples,
-                                 precision_check=a.precision_check)
+                                 precision_check=a.precision_check, closure_operands=a.closure_operands)
     print(json.dumps(res), flush=True)
 
 

@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 NP = oc.N_PARAMS
 
 
-def _model(dev, seed=0, mean=None, std=None, scale=1.0):
-    m = cl.EddyViscosityMLP3D(dev, seed=seed)
+def _model(dev, seed=0, mean=None, std=None, scale=1.0, operands="fp16"):
+    m = cl.EddyViscosityMLP3D(dev, seed=seed, operands=operands)
     if scale != 1.0:
         m.params[:NP] *= scale
         m.mark_updated()
@@ -123,17 +123,18 @@ def _closure_state(seed):
     return F.random_state(19, (6, 10, 16), seed=seed, noise=2e-2, u_amp=0.05)
 
 
+@pytest.mark.parametrize("operands", ["fp16", "bf16"])
 @pytest.mark.parametrize("cs,dtype", [(0.0, torch.float32), (0.17, torch.float32), (0.17, torch.float64)])
-def test_fused_closure_step_matches_oracle(dev, cs, dtype):
+def test_fused_closure_step_matches_oracle(dev, cs, dtype, operands):
     f0 = _closure_state(11)
     tau = 0.56
     # features scale: S ~ Π/(ρτ) for this state
     fin = ref.stream(f0)
     feq, rho_s, _ = ref._feq_safe(fin)
     x = oc.features_from_pi(ref.neq_stress(fin - feq), rho_s, tau).reshape(-1, 7)
-    m = _model(dev, 21, x.mean(0), x.std(0), scale=1.5)
+    m = _model(dev, 21, x.mean(0), x.std(0), scale=1.5, operands=operands)
     params = m.params.cpu().double().numpy()
-    exp_f, exp_nu = oc.fused_mlp_step(f0, tau, params, cs_delta_sq=cs * cs, emulate_mfma=True)
+    exp_f, exp_nu = oc.fused_mlp_step(f0, tau, params, cs_delta_sq=cs * cs, emulate_mfma=True, operands=operands)
     _, nu64 = oc.fused_mlp_step(f0, tau, params, cs_delta_sq=cs * cs, emulate_mfma=False)
     assert np.ptp(exp_nu) > 1e-2, "closure must vary across cells for the check to bite"
 

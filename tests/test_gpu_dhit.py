@@ -49,14 +49,16 @@ def test_deviation_storage_errors_vs_fp64(dev, lat, coll, tau):
     assert e32["u"] < e16["u"] < eb16["u"]
 
 
-def test_fp16_storage_with_learned_closure_vs_fp64(dev):
-    """The FP16-store + closure form stays within 1e-3 (velocity, relative L2) of its FP64 run."""
-    m = EddyViscosityMLP3D(dev, seed=3)
+@pytest.mark.parametrize("operands", ["fp16", "bf16"])
+def test_fp16_storage_with_learned_closure_vs_fp64(dev, operands):
+    """The FP16-store + closure form stays within 1e-3 (velocity, relative L2) of its FP64 run,
+    with either matrix-core operand form of the closure."""
+    m = EddyViscosityMLP3D(dev, seed=3, operands=operands)
     m.set_feature_stats(torch.zeros(7), torch.full((7,), 1e-3))
     ref = _run("D3Q19", "BGK", 0.51, torch.float64, False, dev, closure=m)
     e32 = dg.field_errors(_run("D3Q19", "BGK", 0.51, torch.float32, False, dev, closure=m), ref)
     e16 = dg.field_errors(_run("D3Q19", "BGK", 0.51, torch.float16, True, dev, closure=m), ref, deviation=True)
-    print(f"closure: fp32 {e32}\n fp16-dev {e16}")
+    print(f"closure ({operands}): fp32 {e32}\n fp16-dev {e16}")
     # the closure's inputs carry ~22 bits and its layer-1 activations 11 (fp16): the rare
     # activation rounding flips between storage precisions stay far below the fp16-store error
     assert e32["u"] < 3e-4
