@@ -216,6 +216,33 @@ int tlbm_gradient_closure_tau(int dtype, const void* ux, const void* uy, const v
                               const double* wall_velocity, int nz, int ny, int nx, int model, double coef, double tau,
                               void* tau_out, tlbm_stream_t stream);
 
+/*
+ * Boundary conditions of a post-stream D3Q19 state as a separate op, out of place (f_out != f_in,
+ * contiguous (19, nz, ny, nx), float32 or float64 by dtype).  Every value read is the input's, and
+ * for each cell:
+ *   bc->kind TLBM_BC_CHANNEL_X: x = 0 becomes f_eq(rho of x = 1, (u[0], 0, 0)), x = nx-1 the state
+ *            of x = nx-2 (apply_simple_channel_boundaries_3d, boundaries3d.py:407-443);
+ *   bc->kind TLBM_BC_FAR_FIELD: f_eq(1, u) on the inlet (unless TLBM_FACE_XM_PERIODIC) and on the
+ *            lateral faces named in bc->faces, which win on shared edges; the outlet copies
+ *            x = nx-2 (unless TLBM_FACE_XP_PERIODIC) (far_field_bc_3d, boundaries3d.py:446-537);
+ *   bc->kind TLBM_BC_LID_Y: Zou-He with u_x = u[0] on every cell of y = ny-1, corners included
+ *            (zou_he_moving_lid_3d, boundaries3d.py:324-404);
+ *   bc NULL or TLBM_BC_NONE: no face condition;
+ * then full-way bounce-back where the optional int8 mask_a is non-zero, then where mask_b is
+ * (bounce_back_cells_3d applied twice: a cell in both masks swaps back).  Inlet / outlet kinds
+ * need nx >= 3.
+ */
+int tlbm_boundary_eager(int dtype, const void* f_in, void* f_out, int nz, int ny, int nx,
+                        const tlbm_boundary* bc, const int8_t* mask_a, const int8_t* mask_b,
+                        tlbm_stream_t stream);
+
+/* sphere_mask (boundaries3d.py:29-49): out[z][y][x] = 1 where (x-cx)^2 + (y-cy)^2 + (z-cz)^2 <=
+ * radius^2, else 0, evaluated in float32 as the reference does (centre rounded to float32, squares
+ * and sums rounded in turn, radius^2 formed in double and rounded to float32); out holds nz*ny*nx
+ * bytes (a torch.bool tensor's storage). */
+int tlbm_sphere_mask(int nx, int ny, int nz, double cx, double cy, double cz, double radius,
+                     uint8_t* out, tlbm_stream_t stream);
+
 /* f <- f * (*scale) (correct_mass3d, solver3d.py:287-300); scale is a device double. */
 int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, int ny, int nx,
                const double* scale, tlbm_stream_t stream);

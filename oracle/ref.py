@@ -779,12 +779,15 @@ def collide_cumulant_geier_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=
 
 
 def sphere_mask(nx, ny, nz, cx, cy, cz, radius) -> np.ndarray:
-    """``(x−cx)² + (y−cy)² + (z−cz)² ≤ r²`` in float32 (boundaries3d.py:29-49)."""
+    """``(x−cx)² + (y−cy)² + (z−cz)² ≤ r²`` in float32 (boundaries3d.py:29-49): the centre rounded to
+    float32, each square and sum rounded in turn, and r² formed in double and rounded to float32
+    (torch compares a float32 tensor with a Python float in float32)."""
     zz, yy, xx = np.meshgrid(
         np.arange(nz, dtype=np.float32), np.arange(ny, dtype=np.float32),
         np.arange(nx, dtype=np.float32), indexing="ij",
     )
-    return (xx - np.float32(cx)) ** 2 + (yy - np.float32(cy)) ** 2 + (zz - np.float32(cz)) ** 2 <= np.float32(radius) ** 2
+    d = (xx - np.float32(cx)) ** 2 + (yy - np.float32(cy)) ** 2 + (zz - np.float32(cz)) ** 2
+    return d <= np.float32(float(radius) * float(radius))
 
 
 def obstacle_force(f_streamed: np.ndarray, solid: np.ndarray):

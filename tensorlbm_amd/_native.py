@@ -112,6 +112,9 @@ _SIGNATURES = {
     "tlbm_moment_sums": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
     "tlbm_obstacle_force": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_scale": ([_I, _I, _VP, _VP, _I, _I, _I, _VP, _VP], _I),
+    "tlbm_boundary_eager": ([_I, _VP, _VP, _I, _I, _I, ctypes.POINTER(Boundary), _VP, _VP, _VP], _I),
+    "tlbm_sphere_mask": ([_I, _I, _I, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP,
+                          _VP], _I),
     "tlbm_gradient_closure_tau": ([_I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, ctypes.c_double, ctypes.c_double,
                                    _VP, _VP], _I),
     "tlbm_smagorinsky_tau_sum": ([_I, _I, _VP, _VP, _I, _I, _I, ctypes.c_double, ctypes.c_double, _VP, _VP], _I),

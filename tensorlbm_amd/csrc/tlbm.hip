@@ -778,6 +778,72 @@ __global__ __launch_bounds__(256) void k_bounce_back(const T* fin, T* fout, cons
 }
 
 // ---------------------------------------------------------------------------
+// boundary conditions as separate ops (eager forms of the fused boundaries)
+// ---------------------------------------------------------------------------
+
+// One cell of a post-stream D3Q19 state, out of place: the face conditions of `kind`, then full-way
+// bounce-back where mask_a is set, then where mask_b is (a cell in both swaps twice).  Every value
+// read comes from the input, i.e. the state before any condition: the inlet's density from x = 1
+// and the outlet's copy of x = nx - 2 are taken before the lateral far-field faces are written, as
+// in boundaries3d.py:407-443 / 446-537 (nx >= 3 is checked on the host).
+template <typename T>
+__global__ __launch_bounds__(256) void k_boundary(const T* fin, T* fout, int nz, int ny, int nx, int kind, int faces,
+                                                  T bux, T buy, T buz, const int8_t* mask_a, const int8_t* mask_b) {
+  using L = D3Q19;
+  constexpr int Q = L::Q;
+  const uint64_t N = uint64_t(nz) * ny * nx;
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int x = int(i % uint64_t(nx));
+  const int y = int((i / uint64_t(nx)) % uint64_t(ny));
+  const int z = int(i / (uint64_t(nx) * ny));
+  T f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = fin[uint64_t(q) * N + i];
+  if (kind == TLBM_BC_CHANNEL_X) {
+    if (x == 0) {  // equilibrium inlet, u = (u_in, 0, 0), density of x = 1
+      T rho1 = T(0);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) rho1 += fin[uint64_t(q) * N + i + 1];
+      equilibrium<L>(rho1, bux, T(0), T(0), f);
+    } else if (x == nx - 1) {  // zero-gradient outlet
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = fin[uint64_t(q) * N + i - 1];
+    }
+  } else if (kind == TLBM_BC_FAR_FIELD) {
+    const bool lateral = (y == 0 && (faces & TLBM_FACE_YM)) || (y == ny - 1 && (faces & TLBM_FACE_YP)) ||
+                         (z == 0 && (faces & TLBM_FACE_ZM)) || (z == nz - 1 && (faces & TLBM_FACE_ZP));
+    if (lateral || (x == 0 && !(faces & TLBM_FACE_XM_PERIODIC))) {
+      equilibrium<L>(T(1), bux, buy, buz, f);  // free stream, rho = 1
+    } else if (x == nx - 1 && !(faces & TLBM_FACE_XP_PERIODIC)) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) f[q] = fin[uint64_t(q) * N + i - 1];
+    }
+  } else if (kind == TLBM_BC_LID_Y) {
+    if (y == ny - 1) zou_he_lid(f, bux);  // every cell of the lid plane, corners included
+  }
+  const bool swap = (mask_a != nullptr && mask_a[i] != 0) != (mask_b != nullptr && mask_b[i] != 0);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) fout[uint64_t(q) * N + i] = swap ? f[L::opp[q]] : f[q];
+}
+
+// sphere_mask (boundaries3d.py:29-49) in float32 as the reference evaluates it: centre and
+// coordinates rounded to float32, each square and each sum rounded in turn, against radius^2
+// formed in double and rounded to float32
+__global__ __launch_bounds__(256) void k_sphere_mask(int nx, int ny, int nz, float cx, float cy, float cz, float r2,
+                                                     uint8_t* out) {
+#pragma clang fp contract(off)
+  const uint64_t n = uint64_t(nz) * ny * nx;
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float dx = float(int(i % uint64_t(nx))) - cx;
+  const float dy = float(int((i / uint64_t(nx)) % uint64_t(ny))) - cy;
+  const float dz = float(int(i / (uint64_t(nx) * ny))) - cz;
+  const float d = dx * dx + dy * dy + dz * dz;
+  out[i] = d <= r2 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // gradient eddy-viscosity closures (WALE, Vreman) -> per-cell tau_eff
 // ---------------------------------------------------------------------------
 
@@ -1846,6 +1912,47 @@ int tlbm_scale(int lattice, int store, void* f, const int64_t* strides, int nz, 
       return check_launch("k_scale");
     });
   });
+}
+
+int tlbm_boundary_eager(int dtype, const void* f_in, void* f_out, int nz, int ny, int nx, const tlbm_boundary* bc,
+                        const int8_t* mask_a, const int8_t* mask_b, tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (nz < 1 || ny < 1 || nx < 1) return fail(TLBM_ERR_SHAPE, "grid dims must be >= 1 (got %d,%d,%d)", nz, ny, nx);
+  const uint64_t n = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  if (n >= (uint64_t(1) << 31)) return fail(TLBM_ERR_SHAPE, "grid of %llu cells exceeds 2^31", (unsigned long long)n);
+  if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "null pointer");
+  if (f_in == f_out) return fail(TLBM_ERR_ALIAS, "f_in and f_out alias; the boundary op writes a new state");
+  const int kind = bc != nullptr ? bc->kind : TLBM_BC_NONE;
+  if (kind != TLBM_BC_NONE && kind != TLBM_BC_CHANNEL_X && kind != TLBM_BC_FAR_FIELD && kind != TLBM_BC_LID_Y)
+    return fail(TLBM_ERR_INVALID, "unknown boundary kind %d", kind);
+  if ((kind == TLBM_BC_CHANNEL_X || kind == TLBM_BC_FAR_FIELD) && nx < 3)
+    return fail(TLBM_ERR_SHAPE, "inlet / outlet boundaries need nx >= 3 (got %d)", nx);
+  const int faces = bc != nullptr ? bc->faces : 0;
+  const double u0 = bc != nullptr ? bc->u[0] : 0.0, u1 = bc != nullptr ? bc->u[1] : 0.0,
+               u2 = bc != nullptr ? bc->u[2] : 0.0;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == TLBM_STORE_F32) {
+    hipLaunchKernelGGL(k_boundary<float>, dim3(blocks_for(n)), dim3(256), 0, s, (const float*)f_in, (float*)f_out, nz,
+                       ny, nx, kind, faces, float(u0), float(u1), float(u2), mask_a, mask_b);
+  } else if (dtype == TLBM_STORE_F64) {
+    hipLaunchKernelGGL(k_boundary<double>, dim3(blocks_for(n)), dim3(256), 0, s, (const double*)f_in,
+                       (double*)f_out, nz, ny, nx, kind, faces, u0, u1, u2, mask_a, mask_b);
+  } else {
+    return fail(TLBM_ERR_DTYPE, "boundary ops take float32 or float64 populations");
+  }
+  return check_launch("k_boundary");
+}
+
+int tlbm_sphere_mask(int nx, int ny, int nz, double cx, double cy, double cz, double radius, uint8_t* out,
+                     tlbm_stream_t stream) {
+  g_err[0] = 0;
+  if (nz < 1 || ny < 1 || nx < 1) return fail(TLBM_ERR_SHAPE, "grid dims must be >= 1 (got %d,%d,%d)", nz, ny, nx);
+  if (!out) return fail(TLBM_ERR_INVALID, "null pointer");
+  const uint64_t n = uint64_t(nz) * uint64_t(ny) * uint64_t(nx);
+  if (n >= (uint64_t(1) << 31)) return fail(TLBM_ERR_SHAPE, "grid of %llu cells exceeds 2^31", (unsigned long long)n);
+  hipLaunchKernelGGL(k_sphere_mask, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, nx, ny, nz, float(cx),
+                     float(cy), float(cz), float(radius * radius), out);
+  return check_launch("k_sphere_mask");
 }
 
 int tlbm_gradient_closure_tau(int dtype, const void* ux, const void* uy, const void* uz, const int8_t* solid,

@@ -41,7 +41,8 @@ __all__ = [
     "collide_smagorinsky_mrt3d", "stream",
     "stream3d", "stream3d_roll", "stream27", "stream27_roll", "correct_mass", "correct_mass3d",
     "correct_mass27", "bounce_back_cells", "bounce_back_cells_3d", "bounce_back_cells_27",
-    "compute_obstacle_forces_3d", "moment_sums", "gradient_closure_tau",
+    "compute_obstacle_forces_3d", "moment_sums", "gradient_closure_tau", "collide_bgk_fused", "collide_bgk_matmul",
+    "apply_simple_channel_boundaries_3d", "far_field_bc_3d", "zou_he_moving_lid_3d", "sphere_mask",
 ]
 
 _EAGER = (torch.float32, torch.float64)
@@ -180,6 +181,19 @@ def collide_bgk(f, tau):
     """D2Q9 BGK ``f − (f − f_eq)/τ`` (solver.py:67-71)."""
     return _collide(f, 9, nat.make_collision(nat.COLL_BGK, _tau(tau, "collide_bgk"), eq_rho_floor=False),
                     "collide_bgk")
+
+
+def collide_bgk_fused(f, tau):
+    """D2Q9 BGK in one pass (solver.py:74-117): the same update as ``collide_bgk``, one kernel here too."""
+    return _collide(f, 9, nat.make_collision(nat.COLL_BGK, _tau(tau, "collide_bgk_fused"), eq_rho_floor=False),
+                    "collide_bgk_fused")
+
+
+def collide_bgk_matmul(f, tau):
+    """D2Q9 BGK with moments by matrix products (solver.py:194-246): the same update as ``collide_bgk``;
+    the moments of nine populations are a few adds per cell, no GEMM."""
+    return _collide(f, 9, nat.make_collision(nat.COLL_BGK, _tau(tau, "collide_bgk_matmul"), eq_rho_floor=False),
+                    "collide_bgk_matmul")
 
 
 def collide_bgk3d(f, tau):
@@ -469,6 +483,62 @@ def bounce_back_cells_3d(f, mask, f_pre=None):
 def bounce_back_cells_27(f, mask):
     """D3Q27 full-way swap (boundaries_d3q27.py:45-60)."""
     return _bounce(f, mask, 27, "bounce_back_cells_27")
+
+
+def _mask_arg(mask, f, what):
+    if mask is None:
+        return None
+    if tuple(mask.shape) != tuple(f.shape[1:]):
+        raise ValueError(f"{what}: mask shape {tuple(mask.shape)} must match {tuple(f.shape[1:])}")
+    return mask.to(device=f.device, dtype=torch.int8).contiguous()
+
+
+def _boundary(f, bc, what, mask_a=None, mask_b=None):
+    f, dims = _pops(f, 19, what)
+    dt = _eager_dtype(f, what)
+    ma, mb = _mask_arg(mask_a, f, what), _mask_arg(mask_b, f, what)
+    out = torch.empty_like(f)
+    rc = nat.lib().tlbm_boundary_eager(dt, f.data_ptr(), out.data_ptr(), *dims, bc, nat.ptr(ma), nat.ptr(mb),
+                                       nat.stream_of(f))
+    nat.check(rc, what)
+    return out
+
+
+def apply_simple_channel_boundaries_3d(f, u_in, wall_mask, obstacle_mask):
+    """Equilibrium inlet at x = 0 (u = (u_in, 0, 0), ρ of x = 1), zero-gradient outlet at x = nx−1,
+    then bounce-back on ``wall_mask`` and on ``obstacle_mask`` (boundaries3d.py:407-443).
+
+    Returns a new tensor; the input is not modified (the reference writes its inlet and outlet
+    planes into ``f`` before bouncing back into a new tensor).
+    """
+    bc = nat.make_boundary(nat.BC_CHANNEL_X, (float(u_in), 0.0, 0.0))
+    return _boundary(f, bc, "apply_simple_channel_boundaries_3d", wall_mask, obstacle_mask)
+
+
+def far_field_bc_3d(f, u_in, obstacle_mask=None, uy=0.0, uz=0.0, bc_config=None):
+    """Free-stream equilibrium (ρ = 1, u = (u_in, uy, uz)) on the inlet and the far-field lateral faces,
+    zero-gradient outlet, then bounce-back on ``obstacle_mask`` (boundaries3d.py:446-537).  ``bc_config``
+    takes ``far_field_faces`` / ``periodic_faces`` as there; None means all four lateral faces."""
+    bc = nat.make_boundary(nat.BC_FAR_FIELD, (float(u_in), float(uy), float(uz)), nat.far_field_faces(bc_config))
+    return _boundary(f, bc, "far_field_bc_3d", obstacle_mask)
+
+
+def zou_he_moving_lid_3d(f, u_lid, uy_lid=0.0, uz_lid=0.0):
+    """Zou-He moving lid with u = (u_lid, 0, 0) on every cell of y = ny−1 (boundaries3d.py:324-404)."""
+    if uy_lid != 0.0 or uz_lid != 0.0:
+        raise NotImplementedError("zou_he_moving_lid_3d supports uy = uz = 0 (the reference's lid-driven cavity case)")
+    bc = nat.make_boundary(nat.BC_LID_Y, (float(u_lid), 0.0, 0.0))
+    return _boundary(f, bc, "zou_he_moving_lid_3d")
+
+
+def sphere_mask(nx, ny, nz, cx, cy, cz, radius, device):
+    """Boolean (nz, ny, nx) mask of a sphere (boundaries3d.py:29-49), built on the GPU in float32 as there."""
+    out = torch.empty((int(nz), int(ny), int(nx)), dtype=torch.bool, device=device)
+    nat.require_gpu(out, "sphere_mask")
+    rc = nat.lib().tlbm_sphere_mask(int(nx), int(ny), int(nz), float(cx), float(cy), float(cz), float(radius),
+                                    out.data_ptr(), nat.stream_of(out))
+    nat.check(rc, "sphere_mask")
+    return out
 
 
 def moment_sums(f, *, deviation: bool = False) -> torch.Tensor:

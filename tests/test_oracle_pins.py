@@ -15,6 +15,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+import torch
 
 from oracle import cfast, ref
 from oracle.lattice import D2Q9, D3Q19, D3Q27
@@ -617,3 +618,22 @@ def test_smagorinsky_mrt_uniform_field_equals_mrt_and_conserves(lat):
     idx = (slice(None), 2, 3) if q == 9 else (slice(None), 1, 2, 3)
     cell = f[(slice(None),) + tuple(slice(i, i + 1) for i in idx[1:])]
     np.testing.assert_allclose(out[idx], ref.collide_mrt(cell, float(te[idx[1:]])).reshape(-1), atol=1e-15)
+
+
+@pytest.mark.parametrize("radius,centre,shape", [
+    (5.05, (5.05, 4.0, 4.0), (12, 12, 14)), (7.7, (5.0, 4.3, 4.0), (12, 12, 14)), (3.3, (5.2, 4.5, 4.0), (12, 12, 14)),
+    (24.0, (64.0, 40.0, 40.0), (81, 80, 130))])
+def test_sphere_mask_follows_torch_float32_semantics(radius, centre, shape):
+    """The reference builds the mask from float32 coordinate tensors and Python floats
+    (boundaries3d.py:45-49); torch rounds the centre and r² (a double) to float32.  In the first
+    three cases fp32(r)² and fp32(r²) differ and some cells lie between them, so the threshold's
+    rounding decides those cells."""
+    cx, cy, cz = centre
+    nz, ny, nx = shape
+    zz, yy, xx = torch.meshgrid(torch.arange(nz, dtype=torch.float32), torch.arange(ny, dtype=torch.float32),
+                                torch.arange(nx, dtype=torch.float32), indexing="ij")
+    d = (xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2
+    want = (d <= radius**2).numpy()
+    np.testing.assert_array_equal(ref.sphere_mask(nx, ny, nz, cx, cy, cz, radius), want)
+    if radius != 24.0:  # the case bites: squaring in float32 first would flip cells
+        assert ((d <= float(np.float32(radius) ** 2)).numpy() != want).any()
