@@ -298,6 +298,55 @@ int tlbm_mlp_train_step_noalloc(float* params, const float* x, const float* t, i
                                 float clip_norm, float* loss_out, float* grad_out,
                                 tlbm_stream_t stream);
 
+/*
+ * The reference's 2-D learned closure (EddyViscosityMLP, ai/model.py:19-92): an MLP in -> H -> ... -> H -> 1
+ * with hidden activation tanh / relu / gelu (exact erf form) and a Softplus(beta = 10) output.  Parameters are
+ * one fp32 vector in nn.Sequential order -- per hidden layer W [H][in] then b [H], then the output w [H] and
+ * b [1] -- followed by the input normalisation mean [in] and std [in] (inputs (x - mean) / std).
+ * Built for hidden_features 8, 16, 32, 64; hidden_layers 1..4; in_features 1..8 (3 for the strain features).
+ */
+#define TLBM_ACT_TANH 0
+#define TLBM_ACT_RELU 1
+#define TLBM_ACT_GELU 2
+typedef struct tlbm_mlp2d_arch {
+  int32_t in_features, hidden_features, hidden_layers, activation;
+} tlbm_mlp2d_arch;
+
+/* number of trainable parameters (the normalisation's 2 * in_features follow them); -1 if not built */
+int64_t tlbm_mlp2d_nparam(const tlbm_mlp2d_arch* arch);
+
+/* y[n] = MLP(x[n][in]) in fp32 (EddyViscosityMLP.forward, ai/model.py:83-92) */
+int tlbm_mlp2d_forward(const tlbm_mlp2d_arch* arch, const float* params, const float* x, int64_t n,
+                       float* y, tlbm_stream_t stream);
+
+/* periodic central-difference strain of a (ny, nx) velocity field (strain_rate_tensor_2d,
+ * ai/dataset.py:41-72): feats[ny*nx][3] = S_xx, S_yy, S_xy; target (optional) = c_s^2 |S|,
+ * |S| = sqrt(2 S:S) (extract_les_samples_2d, ai/dataset.py:89-127) */
+int tlbm_strain2d(const float* ux, const float* uy, int ny, int nx, float* feats, float* target,
+                  double c_s, tlbm_stream_t stream);
+
+/* per cell of the velocity field: mode 0 nu_t = max(MLP(S), 0) (predict_nu_t_2d); mode 1
+ * tau_eff = max(tau + 3 nu_t, 0.501) (predict_tau_eff_2d, ai/inference.py:53-65); out (ny, nx) */
+int tlbm_mlp2d_nu_t(const tlbm_mlp2d_arch* arch, const float* params, const float* ux, const float* uy,
+                    int ny, int nx, int mode, double tau, float* out, tlbm_stream_t stream);
+
+/* collide_ai_les_bgk (ai/inference.py:68-92): f_out = f - (f - f_eq(rho, u)) / tau_eff with tau_eff of
+ * each cell's strain; rho, ux, uy are tlbm_macroscopic of f (D2Q9, float32, contiguous (9, ny, nx)) */
+int tlbm_ai_les_bgk(const tlbm_mlp2d_arch* arch, const float* params, const float* f_in, float* f_out,
+                    const float* rho, const float* ux, const float* uy, int ny, int nx, double tau,
+                    tlbm_stream_t stream);
+
+/* MSE gradient partials: partials[ceil(n / 64)][nparam + 2] (last columns: loss share, sum |error|);
+ * hidden_features * hidden_layers at most 120 */
+int tlbm_mlp2d_grad(const tlbm_mlp2d_arch* arch, const float* params, const float* x, const float* t,
+                    int64_t n, float* partials, tlbm_stream_t stream);
+
+/* sum the partials in block order, clip the global gradient norm (<= 0: none), one Adam step
+ * (lr = 0: only reduce); stats[0] = MSE, stats[1] = sum |error| (stats may be NULL) */
+int tlbm_mlp2d_adam(const tlbm_mlp2d_arch* arch, float* params, const float* partials, int nblocks,
+                    float* m, float* v, int step, float lr, float beta1, float beta2, float eps,
+                    float clip_norm, float* stats, tlbm_stream_t stream);
+
 /* block average by an integer factor (_coarsen_mean, ai/pipeline.py:93-107, in 3-D):
  * out[10][nz/f][ny/f][nx/f] = rho, u_x, u_y, u_z, <uxux>, <uyuy>, <uzuz>, <uxuy>, <uxuz>, <uyuz> */
 int tlbm_coarsen_moments(int lattice, int store, const void* f, const int64_t* strides, int nz,

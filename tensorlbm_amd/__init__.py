@@ -38,6 +38,15 @@ from .checkpoint import (
     triton_fused_state_dict,
 )
 from .closure import EddyViscosityMLP3D, TrainConfig3D, train_eddy_viscosity_model_3d
+from .closure2d import (
+    EddyViscosityMLP,
+    collide_ai_les_bgk,
+    extract_les_samples_2d,
+    predict_nu_t_2d,
+    predict_tau_eff_2d,
+    strain_rate_tensor_2d,
+    train_eddy_viscosity_model,
+)
 from .diagnostics import frontal_area, obstacle_drag_coefficient, voxel_wetted_area
 from .executor import LBMStepExecutor
 from .ops import *  # noqa: F401,F403
@@ -56,6 +65,8 @@ TritonFusedSolver3D = FusedSolver3D
 __version__ = "0.1.0"
 
 __all__ = list(_ops_all) + [
+    "EddyViscosityMLP", "collide_ai_les_bgk", "extract_les_samples_2d", "predict_nu_t_2d", "predict_tau_eff_2d",
+    "strain_rate_tensor_2d", "train_eddy_viscosity_model",
     "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
     "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
     "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",

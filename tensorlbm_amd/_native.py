@@ -127,6 +127,14 @@ _SIGNATURES = {
                        ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
     "tlbm_mlp_train_step_noalloc": ([_VP, _VP, _VP, _I64, _VP, _VP, _I, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, _VP, _VP, _VP], _I),
+    "tlbm_mlp2d_nparam": ([_VP], _I64),
+    "tlbm_mlp2d_forward": ([_VP, _VP, _VP, _I64, _VP, _VP], _I),
+    "tlbm_strain2d": ([_VP, _VP, _I, _I, _VP, _VP, ctypes.c_double, _VP], _I),
+    "tlbm_mlp2d_nu_t": ([_VP, _VP, _VP, _VP, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
+    "tlbm_ai_les_bgk": ([_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _I, ctypes.c_double, _VP], _I),
+    "tlbm_mlp2d_grad": ([_VP, _VP, _VP, _VP, _I64, _VP, _VP], _I),
+    "tlbm_mlp2d_adam": ([_VP, _VP, _VP, _I, _VP, _VP, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, ctypes.c_float, _VP, _VP], _I),
     "tlbm_coarsen_moments": ([_I, _I, _VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "tlbm_sgs_samples": ([_VP, _I, _I, _I, _VP, _VP, _VP], _I),
     "tlbm_cavities_layout": ([_VP, _I, _VP, _VP], _I),

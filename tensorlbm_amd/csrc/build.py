@@ -33,7 +33,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-SOURCES = ("tlbm.hip", "closure.hip", "cavities.hip")
+SOURCES = ("tlbm.hip", "closure.hip", "closure2d.hip", "cavities.hip")
 OBJ_DIR = PKG / "_build"
 TORCH_OPS_SRC = CSRC / "torch_ops.cpp"
 TORCH_OPS_OUT = PKG / "libtlbm_torch.so"

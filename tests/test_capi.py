@@ -125,3 +125,19 @@ def test_noalloc_training_step_rejects_oversized_batches_before_any_launch():
     assert b"no-allocation limit" in lib.tlbm_last_error()
     assert lib.tlbm_mlp_train_step_noalloc(p, p, p, 0, *args) == nat.ERR_SHAPE
     assert lib.tlbm_mlp_train_step_noalloc(None, p, p, 64, *args) == nat.ERR_INVALID
+
+
+def test_mlp2d_architecture_checks_without_a_device():
+    from tensorlbm_amd import closure2d as c2
+
+    lib = nat.lib()
+    assert lib.tlbm_mlp2d_nparam(ctypes.byref(c2.ModelArch().native())) == 3 * 16 + 16 + 16 * 16 + 16 + 16 + 1
+    assert lib.tlbm_mlp2d_nparam(ctypes.byref(c2.ModelArch(5, 64, 1, "gelu").native())) == 5 * 64 + 64 + 64 + 1
+    assert lib.tlbm_mlp2d_nparam(ctypes.byref(c2.ModelArch(hidden_features=20).native())) == -1
+    assert b"hidden_features" in lib.tlbm_last_error()
+    assert lib.tlbm_mlp2d_nparam(ctypes.byref(c2.ModelArch(n_hidden_layers=5).native())) == -1
+    with pytest.raises(ValueError, match="Unsupported activation"):
+        c2.ModelArch(activation="swish").native()
+    p = ctypes.c_void_p(0x1000)
+    arch = c2.ModelArch(3, 64, 2).native()  # 64 x 2 > 120: training's LDS would exceed 64 KB
+    assert lib.tlbm_mlp2d_grad(ctypes.byref(arch), p, p, p, 100, p, None) == nat.ERR_UNSUPPORTED

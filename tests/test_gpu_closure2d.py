@@ -1,0 +1,297 @@
+"""The reference's 2-D learned closure on the GPU kernels, against torch's own modules on the CPU.
+
+The reference's ``EddyViscosityMLP`` is ``nn.Sequential`` of ``nn.Linear`` / ``nn.Tanh`` (``ReLU``,
+``GELU``) / ``nn.Softplus(beta=10)`` after a ``(x - mean) / std`` normalisation (ai/model.py:19-92),
+trained with ``nn.MSELoss``, ``clip_grad_norm_`` and ``optim.Adam`` (ai/train.py:310-464).  The same
+torch modules, built here in float64 on the CPU from the HIP model's state dict, are the reference
+for the forward pass, the gradient, an Adam step and a short training run; the strain and LES
+collision follow ai/dataset.py:41-127 and ai/inference.py:26-92 in torch float32.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+from torch import nn
+
+import tensorlbm_amd as tl
+from oracle import ref
+from tensorlbm_amd import closure2d as c2
+from tests import _fields as F
+
+pytestmark = pytest.mark.gpu
+
+ARCHS = [c2.ModelArch(3, 16, 2, "tanh"), c2.ModelArch(3, 8, 1, "relu"), c2.ModelArch(3, 32, 3, "gelu"),
+         c2.ModelArch(5, 64, 1, "tanh")]
+
+
+def _torch_model(m: c2.EddyViscosityMLP, dtype=torch.float64):
+    """The reference's module structure with m's weights (CPU)."""
+    a = m.arch
+    act = {"tanh": nn.Tanh, "relu": nn.ReLU, "gelu": nn.GELU}[a.activation]
+    layers, d = [], a.in_features
+    for _ in range(a.n_hidden_layers):
+        layers += [nn.Linear(d, a.hidden_features), act()]
+        d = a.hidden_features
+    layers += [nn.Linear(d, 1), nn.Softplus(beta=10.0)]
+
+    class Ref(nn.Module):
+        def __init__(self):
+            super().__init__()
+            self.register_buffer("feature_mean", torch.zeros(a.in_features))
+            self.register_buffer("feature_std", torch.ones(a.in_features))
+            self.net = nn.Sequential(*layers)
+
+        def forward(self, x):
+            return self.net((x - self.feature_mean) / self.feature_std)
+
+    r = Ref()
+    r.load_state_dict(m.state_dict())
+    return r.to(dtype)
+
+
+def _model(arch, seed, dev="cuda:0"):
+    torch.manual_seed(seed)
+    m = c2.EddyViscosityMLP(arch, device=dev)
+    g = torch.Generator().manual_seed(seed + 1)
+    m.set_feature_stats(0.01 * torch.randn(arch.in_features, generator=g), 0.05 + torch.rand(arch.in_features,
+                                                                                           generator=g) * 0.05)
+    return m
+
+
+def _features(n, arch, seed):
+    g = torch.Generator().manual_seed(seed)
+    return 0.05 * torch.randn(n, arch.in_features, generator=g)
+
+
+@pytest.mark.parametrize("arch", ARCHS, ids=lambda a: f"{a.in_features}-{a.hidden_features}x{a.n_hidden_layers}-{a.activation}")
+def test_forward_matches_torch(arch):
+    m = _model(arch, 3)
+    x = _features(3000, arch, 4)
+    got = m(x.cuda()).cpu().double()
+    want = _torch_model(m)(x.double()).detach()
+    assert got.shape == (3000, 1)
+    np.testing.assert_allclose(got.numpy(), want.numpy(), rtol=2e-5, atol=1e-7)
+
+
+def test_initial_weights_are_nn_linear_draws():
+    """After torch.manual_seed(s) a new model holds the weights nn.Linear would draw, layer by layer."""
+    arch = c2.ModelArch()
+    torch.manual_seed(11)
+    m = c2.EddyViscosityMLP(arch)
+    torch.manual_seed(11)
+    l1, l2, l3 = nn.Linear(3, 16), nn.Linear(16, 16), nn.Linear(16, 1)
+    sd = m.state_dict()
+    for k, lin in ((0, l1), (2, l2), (4, l3)):
+        assert torch.equal(sd[f"net.{k}.weight"], lin.weight.detach())
+        assert torch.equal(sd[f"net.{k}.bias"], lin.bias.detach())
+
+
+def test_save_and_load_interchange_with_torch_modules(tmp_path):
+    m = _model(c2.ModelArch(3, 32, 2, "relu"), 5)
+    p = c2.save_model(m, tmp_path / "m.pt")
+    assert (tmp_path / "m.pt.json").exists()
+    ref_module = _torch_model(m, torch.float32)
+    ref_module.load_state_dict(torch.load(p, weights_only=True))  # the reference's loader path
+    back = c2.load_model(p)
+    assert back.arch == m.arch and torch.equal(back.params, m.params)
+    x = _features(500, m.arch, 6)
+    np.testing.assert_allclose(back(x.cuda()).cpu().numpy(), ref_module(x).detach().numpy(), rtol=2e-5, atol=1e-7)
+
+
+def _velocity(ny, nx, seed):
+    f = F.random_state(9, (ny, nx), seed=seed, noise=0.0, u_amp=0.04)
+    _, ux, uy = ref.macroscopic(f)[:3]
+    return torch.from_numpy(ux).float(), torch.from_numpy(uy).float()
+
+
+def _strain_torch(ux, uy):
+    dudx = 0.5 * (torch.roll(ux, -1, dims=-1) - torch.roll(ux, 1, dims=-1))
+    dudy = 0.5 * (torch.roll(ux, -1, dims=-2) - torch.roll(ux, 1, dims=-2))
+    dvdx = 0.5 * (torch.roll(uy, -1, dims=-1) - torch.roll(uy, 1, dims=-1))
+    dvdy = 0.5 * (torch.roll(uy, -1, dims=-2) - torch.roll(uy, 1, dims=-2))
+    return dudx, dvdy, 0.5 * (dudy + dvdx)
+
+
+def test_strain_and_samples():
+    ux, uy = _velocity(24, 40, 7)
+    got = c2.strain_rate_tensor_2d(ux.cuda(), uy.cuda())
+    for a, b in zip(got, _strain_torch(ux, uy)):
+        assert torch.equal(a.cpu(), b)
+    mask = torch.zeros(24, 40, dtype=torch.bool)
+    mask[5:9, 10:20] = True
+    feats, target = c2.extract_les_samples_2d(ux.cuda(), uy.cuda(), c_s=0.17, mask=mask.cuda())
+    sxx, syy, sxy = _strain_torch(ux, uy)
+    keep = ~mask.reshape(-1)
+    want_f = torch.stack([sxx, syy, sxy], dim=-1).reshape(-1, 3)[keep]
+    want_t = (0.17 * 0.17 * torch.sqrt(2.0 * (sxx * sxx + syy * syy + 2.0 * sxy * sxy))).reshape(-1, 1)[keep]
+    assert feats.shape == (24 * 40 - 40, 3) and target.shape == (24 * 40 - 40, 1)
+    assert torch.equal(feats.cpu(), want_f)
+    np.testing.assert_allclose(target.cpu().numpy(), want_t.numpy(), rtol=1e-6, atol=0)
+
+
+def test_predictions_and_les_collision():
+    m = _model(c2.ModelArch(), 8)
+    ux, uy = _velocity(20, 36, 9)
+    tm = _torch_model(m)
+    sxx, syy, sxy = _strain_torch(ux, uy)
+    nu_ref = tm(torch.stack([sxx, syy, sxy], -1).reshape(-1, 3).double()).detach().reshape(20, 36).clamp_min(0.0)
+    nu = c2.predict_nu_t_2d(m, ux.cuda(), uy.cuda()).cpu().double()
+    np.testing.assert_allclose(nu.numpy(), nu_ref.numpy(), rtol=2e-5, atol=1e-9)
+    te = c2.predict_tau_eff_2d(m, ux.cuda(), uy.cuda(), 0.52).cpu().double()
+    np.testing.assert_allclose(te.numpy(), torch.clamp_min(0.52 + 3.0 * nu_ref, 0.501).numpy(), rtol=1e-6, atol=0)
+    # the collision: macroscopic, equilibrium with rho, relaxation at tau_eff (inference.py:68-92)
+    f = torch.from_numpy(F.random_state(9, (20, 36), seed=10, noise=3e-3, u_amp=0.04)).float()
+    out = tl.collide_ai_les_bgk(f.cuda(), 0.52, m).cpu().double().numpy()
+    fd = f.double().numpy()
+    rho, vx, vy = ref.macroscopic(fd)[:3]
+    sx, sy, sz = _strain_torch(torch.from_numpy(vx), torch.from_numpy(vy))
+    nu2 = tm(torch.stack([sx, sy, sz], -1).reshape(-1, 3)).detach().reshape(20, 36).clamp_min(0.0)
+    tau_eff = torch.clamp_min(0.52 + 3.0 * nu2, 0.501).numpy()
+    feq = ref.equilibrium(ref.D2Q9, rho, vx, vy)
+    want = fd - (fd - feq) / tau_eff[None]
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-6)
+
+
+def _flat_grads(module):
+    return torch.cat([p.grad.reshape(-1) for p in module.net.parameters()])
+
+
+@pytest.mark.parametrize("arch", ARCHS[:3], ids=lambda a: f"{a.hidden_features}x{a.n_hidden_layers}-{a.activation}")
+def test_gradient_and_adam_step_match_torch(arch):
+    import ctypes
+
+    from tensorlbm_amd import _native as nat
+
+    m = _model(arch, 12)
+    x = _features(1000, arch, 13)
+    t = 1e-3 * torch.rand(1000, generator=torch.Generator().manual_seed(14))
+    tm = _torch_model(m)
+    loss = nn.MSELoss()(tm(x.double()), t.double().reshape(-1, 1))
+    loss.backward()
+    g_ref = _flat_grads(tm)
+    nb = (1000 + 63) // 64
+    partials = torch.empty((nb, m.nparam + 2), device="cuda:0")
+    xd, td = x.cuda().contiguous(), t.cuda().contiguous()
+    lib, na = nat.lib(), ctypes.byref(m._native)
+    nat.check(lib.tlbm_mlp2d_grad(na, m.params.data_ptr(), xd.data_ptr(), td.data_ptr(), 1000, partials.data_ptr(),
+                                  None), "grad")
+    torch.cuda.synchronize()
+    tot = partials.double().sum(0).cpu()
+    g = tot[:m.nparam]
+    scale = g_ref.abs().max().item()
+    np.testing.assert_allclose(g.numpy(), g_ref.numpy(), rtol=0, atol=2e-5 * scale)
+    assert abs(tot[m.nparam].item() - loss.item()) <= 1e-5 * loss.item()
+    # one clipped Adam step against torch.optim.Adam + clip_grad_norm_; eps = 1e-4 on both sides so
+    # that components with |g| near eps (a sign-like step either way) do not amplify fp32 rounding
+    mom, vel, stats = (torch.zeros(m.nparam, device="cuda:0") for _ in range(3))
+    nat.check(lib.tlbm_mlp2d_adam(na, m.params.data_ptr(), partials.data_ptr(), nb, mom.data_ptr(), vel.data_ptr(), 1,
+                                  1e-2, 0.9, 0.999, 1e-4, 0.5 * g_ref.norm().item(), stats.data_ptr(), None), "adam")
+    opt = torch.optim.Adam(tm.net.parameters(), lr=1e-2, eps=1e-4)
+    nn.utils.clip_grad_norm_(tm.net.parameters(), max_norm=0.5 * g_ref.norm().item())
+    opt.step()
+    want = torch.cat([p.detach().reshape(-1) for p in tm.net.parameters()])
+    got = m.params[:m.nparam].double().cpu()
+    np.testing.assert_allclose(got.numpy(), want.numpy(), rtol=0, atol=1e-5)  # lr 1e-2: a step of ~1e-2
+
+
+def _reference_loop(dataset, cfg):
+    """train.py:347-418 in torch (CPU, float32): the loop the GPU training reproduces."""
+    train_ds, val_ds = dataset.split(cfg.val_fraction, seed=cfg.seed)
+    x, y = train_ds.features.float().cpu(), train_ds.targets.float().cpu()
+    torch.manual_seed(cfg.seed)
+    m = c2.EddyViscosityMLP(c2.ModelArch(3, cfg.hidden_features, cfg.n_hidden_layers, cfg.activation))
+    m.set_feature_stats(x.mean(0), x.std(0, unbiased=False).clamp_min(1e-6))
+    tm = _torch_model(m, torch.float32)
+    opt = torch.optim.Adam(tm.net.parameters(), lr=cfg.learning_rate)
+    gen = torch.Generator().manual_seed(cfg.seed)
+    for _ in range(cfg.epochs):
+        perm = torch.randperm(x.shape[0], generator=gen)
+        for i in range(0, x.shape[0], cfg.batch_size):
+            idx = perm[i:i + cfg.batch_size]
+            opt.zero_grad()
+            loss = nn.MSELoss()(tm(x[idx]), y[idx])
+            loss.backward()
+            nn.utils.clip_grad_norm_(tm.net.parameters(), max_norm=cfg.gradient_clip_norm)
+            opt.step()
+    return tm
+
+
+def test_training_follows_the_reference_loop(tmp_path):
+    ux, uy = _velocity(64, 64, 15)
+    feats, target = c2.extract_les_samples_2d(ux.cuda(), uy.cuda(), c_s=0.17)
+    ds = c2.EddyViscosityDataset(feats.cpu(), target.cpu(), 0.17, "random field")
+    cfg = c2.TrainConfig(epochs=2, batch_size=512, learning_rate=3e-3, seed=3)
+    meta = tl.train_eddy_viscosity_model(ds, tmp_path / "model.pt", cfg)
+    for k in ("path", "arch", "config", "history", "final_val_r2", "best_epoch", "n_samples_train", "training_time_s"):
+        assert k in meta
+    assert meta["n_samples_train"] + meta["n_samples_val"] == 64 * 64 and len(meta["history"]) == 2
+    trained = c2.load_model(tmp_path / "model.pt")
+    tm = _reference_loop(ds, cfg)
+    want = torch.cat([p.detach().reshape(-1) for p in tm.net.parameters()])
+    got = trained.params[:trained.nparam].cpu()
+    # same initial weights, minibatches and update rule, float32 on both sides (2 epochs, 16 steps).
+    # Adam moves a parameter whose gradient is near eps by up to lr either way, so parameters are
+    # compared by distribution and the network's outputs tightly.
+    d = (got - want).abs()
+    assert float((d <= 1e-4 * want.abs().max()).float().mean()) >= 0.98, d.max()
+    assert float(d.max()) <= 10 * cfg.learning_rate
+    p_gpu = trained(feats).cpu()
+    p_ref = tm(feats.cpu()).detach()
+    np.testing.assert_allclose(p_gpu.numpy(), p_ref.numpy(), rtol=0, atol=2e-3 * p_ref.abs().max().item())
+
+
+def _sine_velocity(nx, ny):
+    """The smooth shear field the reference's training tests use (tests/test_ai_turbulence.py:43-51)."""
+    yy, xx = torch.meshgrid(torch.arange(ny).float(), torch.arange(nx).float(), indexing="ij")
+    kx, ky = 2.0 * torch.pi / nx, 2.0 * torch.pi / ny
+    return (0.05 + 0.02 * torch.sin(kx * xx) * torch.cos(ky * yy), 0.02 * torch.cos(kx * xx) * torch.sin(ky * yy))
+
+
+def test_training_converges(tmp_path):
+    """The reference's convergence case (tests/test_ai_turbulence.py:204-226): three 24x24 snapshots with
+    1e-3 feature noise, 20 epochs, batch 256, lr 5e-3 -- the last epoch's train MSE is below the first."""
+    torch.manual_seed(0)
+    feats, target = [], []
+    for _ in range(3):
+        f, t = c2.extract_les_samples_2d(*[v.cuda() for v in _sine_velocity(24, 24)])
+        feats.append(f.cpu() + 0.001 * torch.randn(f.shape))
+        target.append(t.cpu())
+    ds = c2.EddyViscosityDataset(torch.cat(feats), torch.cat(target))
+    meta = tl.train_eddy_viscosity_model(ds, tmp_path / "m.pt",
+                                         c2.TrainConfig(epochs=20, batch_size=256, learning_rate=5e-3, seed=0))
+    h = meta["history"]
+    print(f"train MSE {h[0]['train_mse']:.3e} -> {h[-1]['train_mse']:.3e}")
+    assert h[-1]["train_mse"] < h[0]["train_mse"]
+    assert (tmp_path / "m.pt").exists()
+
+
+def test_training_fits_the_smagorinsky_law(tmp_path):
+    """On random-field snapshots the Smagorinsky label is learned to validation R² 0.53 in 40 epochs; the
+    same loop in torch float32 on the CPU reaches 0.57 (softplus(beta=10) starts near 0.07 against targets
+    of 2e-3, so most epochs go to pulling the output scale down).  A gradient with a wrong sign or term
+    does not get past R² 0, so the bound is 0.4."""
+    rows = [c2.extract_les_samples_2d(*[v.cuda() for v in _velocity(96, 96, s)], c_s=0.17) for s in range(3)]
+    ds = c2.EddyViscosityDataset(torch.cat([r[0] for r in rows]).cpu(), torch.cat([r[1] for r in rows]).cpu(), 0.17)
+    meta = tl.train_eddy_viscosity_model(ds, tmp_path / "fit.pt", c2.TrainConfig(epochs=40, batch_size=1024,
+                                                                                   learning_rate=3e-3,
+                                                                                   lr_scheduler="cosine"))
+    h = meta["history"]
+    print(f"val R2 {meta['final_val_r2']:.4f} after {len(h)} epochs")
+    assert meta["final_val_r2"] > 0.4
+    assert h[-1]["val_mse"] < 0.5 * h[0]["val_mse"]  # 0.25 on the CPU
+
+
+def test_errors():
+    with pytest.raises(ValueError, match="Unsupported activation"):
+        c2.EddyViscosityMLP(c2.ModelArch(activation="swish"))
+    with pytest.raises(NotImplementedError):
+        c2.EddyViscosityMLP(c2.ModelArch(hidden_features=20))
+    m = c2.EddyViscosityMLP()
+    with pytest.raises(ValueError, match="Expected input with 3 features"):
+        m(torch.zeros(4, 2, device="cuda:0"))
+    with pytest.raises(ValueError, match="Normalization stats"):
+        m.set_feature_stats(torch.zeros(2), torch.ones(2))
+    with pytest.raises(ValueError, match="float32"):
+        tl.collide_ai_les_bgk(torch.zeros(9, 4, 4, dtype=torch.float64, device="cuda:0"), 0.6, m)
