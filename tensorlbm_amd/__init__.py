@@ -51,6 +51,7 @@ from .diagnostics import frontal_area, obstacle_drag_coefficient, voxel_wetted_a
 from .executor import LBMStepExecutor
 from .ops import *  # noqa: F401,F403
 from .ops import __all__ as _ops_all
+from .pipeline2d import AIPipelineResult, run_ai_dns_pipeline, run_ai_les_pipeline
 from .precision import PrecisionPolicy, cast_to_compute, cast_to_store
 from .reporters import CallbackReporter, StepContext, ThroughputReporter
 
@@ -66,8 +67,8 @@ __version__ = "0.1.0"
 
 __all__ = list(_ops_all) + [
     "EddyViscosityMLP", "collide_ai_les_bgk", "extract_les_samples_2d", "predict_nu_t_2d", "predict_tau_eff_2d",
-    "strain_rate_tensor_2d", "train_eddy_viscosity_model",
-    "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
+    "strain_rate_tensor_2d", "train_eddy_viscosity_model", "AIPipelineResult", "run_ai_les_pipeline",
+    "run_ai_dns_pipeline", "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
     "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
     "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",
     "TritonFusedSolver3D", "LBMStepExecutor", "BatchedCavities", "EddyViscosityMLP3D", "TrainConfig3D",
