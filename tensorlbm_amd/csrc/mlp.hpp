@@ -50,11 +50,19 @@ constexpr int OISTD = OMEAN + NIN;   // 1352
 constexpr int NTOTAL = OISTD + NIN;  // 1359
 constexpr float BETA = 10.0f;
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 // Matrix-core operand blob, built once per weight update by tlbm_mlp_pack.
 struct Packed {
-  uint16_t a1[64][8];     // layer-1 A (fp16 bits): W1 row (lane % 32), inputs 0..6 (both K halves),
-                          // element 7: b1 hi (lanes < 32) / lo (lanes >= 32)
-  uint16_t a2[2][64][8];  // layer-2 A (fp16 bits), K-steps 0 and 1, columns in C-layout order
+  u16x8 a1[64];     // layer-1 A (fp16 bits): W1 row (lane % 32), inputs 0..6 (both K halves),
+                    // element 7: b1 hi (lanes < 32) / lo (lanes >= 32)
+  u16x8 a2[2][64];  // layer-2 A (fp16 bits), K-steps 0 and 1, columns in C-layout order
   float b1c[2][16];       // layer-1 bias, C layout per lane half (reference copy; the tile takes it via K)
   float b2c[2][16];       // layer-2 bias
   float w3c[2][16];       // output weights
@@ -64,19 +72,14 @@ struct Packed {
   float pad;
 };
 static_assert(sizeof(Packed) == 3520, "packed MLP blob layout");
+// the blob as staged in a workgroup's LDS
+typedef __attribute__((address_space(3))) const Packed* LdsPacked;
 
 // hidden unit held by C-tile element r of a lane in half h (= lane / 32)
 __host__ __device__ constexpr int c_row(int h, int r) { return 4 * h + (r % 4) + 8 * (r / 4); }
 // W2 column used by layer-2 K-step s, lane half h, element e
 __host__ __device__ constexpr int w2_col(int s, int h, int e) { return 4 * h + (e % 4) + 16 * s + 8 * (e / 4); }
 
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ f16x8 as_f16x8(u16x8 v) { return __builtin_bit_cast(f16x8, v); }
 
@@ -115,7 +118,8 @@ __device__ __forceinline__ f16x8 act16(const f32x16& c, int s) {
 }
 
 // layer 2 + output for one 32-cell block whose layer-1 tile is c
-__device__ __forceinline__ float block_out(const Packed* P, int h, const f32x16& c, f16x8 a20, f16x8 a21) {
+template <class PP>
+__device__ __forceinline__ float block_out(PP P, int h, const f32x16& c, f16x8 a20, f16x8 a21) {
   f32x16 d;
 #pragma unroll
   for (int r = 0; r < 16; ++r) d[r] = P->b2c[h][r];
@@ -129,8 +133,9 @@ __device__ __forceinline__ float block_out(const Packed* P, int h, const f32x16&
 
 // Eddy viscosity of the calling lane's cell.  Must be called by all 64 lanes
 // of the wave (inactive cells pass any finite features and ignore the result).
-// P points to the packed blob, x to 7 raw features.
-__device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
+// P points to the packed blob (a global or an LDS pointer), x to 7 raw features.
+template <class PP>
+__device__ __forceinline__ float nu_t(PP P, const float (&x)[7]) {
   const int lane = __lane_id();
   const int h = lane >> 5;
   f16x8 hi, lo;
@@ -147,9 +152,9 @@ __device__ __forceinline__ float nu_t(const Packed* P, const float (&x)[7]) {
   const f16x8 got = xor32(h == 0 ? lo : hi);
   const f16x8 b0 = h == 0 ? hi : got;
   const f16x8 b1 = h == 0 ? got : lo;
-  const f16x8 a1 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a1[lane]));
-  const f16x8 a20 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
-  const f16x8 a21 = as_f16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
+  const f16x8 a1 = as_f16x8(P->a1[lane]);
+  const f16x8 a20 = as_f16x8(P->a2[0][lane]);
+  const f16x8 a21 = as_f16x8(P->a2[1][lane]);
   // one 32-cell block at a time keeps a single accumulator tile pair live
   const f32x16 zero = {};
   f32x16 c = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, zero, 0, 0, 0);
@@ -176,7 +181,8 @@ __device__ __forceinline__ void act_bf16(const f32x16& c, int s, bf16x8& hi, bf1
 }
 
 // layer 2 (hi and lo activations against the same W2 tile) + output for one 32-cell block
-__device__ __forceinline__ float block_out_bf16(const Packed* P, int h, const f32x16& c, bf16x8 a20, bf16x8 a21) {
+template <class PP>
+__device__ __forceinline__ float block_out_bf16(PP P, int h, const f32x16& c, bf16x8 a20, bf16x8 a21) {
   f32x16 d;
 #pragma unroll
   for (int r = 0; r < 16; ++r) d[r] = P->b2c[h][r];
@@ -194,7 +200,8 @@ __device__ __forceinline__ float block_out_bf16(const Packed* P, int h, const f3
 }
 
 // nu_t of the bf16 blob (tlbm_mlp_pack_bf16); same calling rules as nu_t
-__device__ __forceinline__ float nu_t_bf16(const Packed* P, const float (&x)[7]) {
+template <class PP>
+__device__ __forceinline__ float nu_t_bf16(PP P, const float (&x)[7]) {
   const int lane = __lane_id();
   const int h = lane >> 5;
   bf16x8 hi, lo;
@@ -208,9 +215,9 @@ __device__ __forceinline__ float nu_t_bf16(const Packed* P, const float (&x)[7])
   const bf16x8 got = xor32(h == 0 ? lo : hi);
   const bf16x8 b0 = h == 0 ? hi : got;
   const bf16x8 b1 = h == 0 ? got : lo;
-  const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a1[lane]));
-  const bf16x8 a20 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[0][lane]));
-  const bf16x8 a21 = as_bf16x8(*reinterpret_cast<const u16x8*>(P->a2[1][lane]));
+  const bf16x8 a1 = as_bf16x8(P->a1[lane]);
+  const bf16x8 a20 = as_bf16x8(P->a2[0][lane]);
+  const bf16x8 a21 = as_bf16x8(P->a2[1][lane]);
   const f32x16 zero = {};
   f32x16 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, zero, 0, 0, 0);
   const float p0 = block_out_bf16(P, h, c, a20, a21);

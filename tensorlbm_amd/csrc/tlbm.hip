@@ -121,8 +121,9 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// pulled population q of cell (x, y, z) (periodic wrap), used off the hot loop
-template <class ST, class L, typename C>
+// pulled population q of cell (x, y, z) (periodic wrap), used off the hot loop;
+// `Uniform` as for gather (z is wave-uniform wherever it is set)
+template <class ST, class L, typename C, bool Uniform = false>
 __device__ __forceinline__ typename ST::C pull_one(const typename ST::T* fin, const KArgs<C>& a, int q, int x, int y,
                                                    int z) {
   int sx = x - L::cx[q], sy = y - L::cy[q], sz = z - L::cz[q];
@@ -130,6 +131,7 @@ __device__ __forceinline__ typename ST::C pull_one(const typename ST::T* fin, co
   sy = sy < 0 ? sy + a.ny : (sy >= a.ny ? sy - a.ny : sy);
   sz = sz < 0 ? sz + a.nz : (sz >= a.nz ? sz - a.nz : sz);
   const typename ST::T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
+  if constexpr (Uniform) base = uniform_ptr(base);
   return ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
 }
 
@@ -187,11 +189,21 @@ __device__ __forceinline__ void own_force_sample(const KArgs<typename Store<S>::
 //
 // `eq_unit(e)` fills e with f_eq(rho = 1, bc_u), from which the inlet takes
 // rho_1 * e (f_eq is linear in rho) and the far-field faces take e itself.
-template <class L, int S, int COLL, bool STREAM, int FEAT, int BCN, int BCK, class Pulled, class EqUnit>
+//
+// The learned closure reads its operand blob from `mlp_blob` when the kernel
+// staged it (an LDS pointer), else from the global blob a.mlp.
+// `before_collision()` runs after the boundary loads and bounce-back, before the
+// collision (the closure kernel issues its next plane's loads there).
+struct NoHook {
+  __device__ void operator()() const {}
+};
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCN, int BCK, class Pulled, class EqUnit,
+          class Blob = const mlp::Packed*, class Hook = NoHook>
 __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], const KArgs<typename Store<S>::C>& a,
                                             int xx, int yy, int z, uint32_t cell, bool solid, bool nb_edge,
                                             double& fx, double& fy, double& fz, const Pulled& pulled,
-                                            const EqUnit& eq_unit) {
+                                            const EqUnit& eq_unit, Blob mlp_blob = nullptr,
+                                            const Hook& before_collision = Hook()) {
   using C = typename Store<S>::C;
   constexpr int Q = L::Q;
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
@@ -265,6 +277,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       }
     }
   }
+  before_collision();
 
   if constexpr (COLL != TLBM_COLL_NONE) {
     const Macro<C> mc = macro<L>(f);
@@ -292,9 +305,15 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       for (int k = 0; k < 6; ++k) xs[k] = float(sc * pi[k]);
       xs[6] = __builtin_amdgcn_sqrtf(2.0f * (xs[0] * xs[0] + xs[1] * xs[1] + xs[2] * xs[2] +
                                              2.0f * (xs[3] * xs[3] + xs[4] * xs[4] + xs[5] * xs[5])));
-      // operands straight from the (cache-resident, 3.5 KB) blob: no staging barrier
-      const mlp::Packed* blob = static_cast<const mlp::Packed*>(a.mlp);
-      const float nu = (FEAT & FEAT_MLP_BF16) != 0 ? mlp::nu_t_bf16(blob, xs) : mlp::nu_t(blob, xs);
+      // operands from the workgroup's LDS copy of the blob where the kernel staged one,
+      // else straight from the (cache-resident, 3.5 KB) global blob
+      float nu;
+      if constexpr (std::is_same_v<Blob, mlp::LdsPacked>) {
+        nu = (FEAT & FEAT_MLP_BF16) != 0 ? mlp::nu_t_bf16(mlp_blob, xs) : mlp::nu_t(mlp_blob, xs);
+      } else {
+        const mlp::Packed* blob = static_cast<const mlp::Packed*>(a.mlp);
+        nu = (FEAT & FEAT_MLP_BF16) != 0 ? mlp::nu_t_bf16(blob, xs) : mlp::nu_t(blob, xs);
+      }
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
@@ -373,6 +392,27 @@ __device__ __forceinline__ void gather(typename Store<S>::C (&f)[L::Q], const ty
     const int sz = src_of(L::cz[q], zd, z, zu);
     const typename ST::T* base = fin + int64_t(q) * a.isq + int64_t(sz) * a.isz;
     f[q] = ST::template get<L>(ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx)), q);
+  }
+}
+
+// The same loads as gather, left in the storage format (a prefetch converts them
+// where they are used, so nothing waits for them at the load), each plane base
+// moved to scalar registers: in a loop over z the compiler otherwise
+// strength-reduces the Q addresses into 64-bit per-lane pointers held live
+// across iterations.
+template <class L, int S>
+__device__ __forceinline__ void gather_raw(typename Store<S>::T (&r)[L::Q], const typename Store<S>::T* fin,
+                                           const KArgs<typename Store<S>::C>& a, int x, int y, int z) {
+  const int xd = wrap_dn(x, a.nx), xu = wrap_up(x, a.nx);
+  const int yd = wrap_dn(y, a.ny), yu = wrap_up(y, a.ny);
+  const int zd = wrap_dn(z, a.nz), zu = wrap_up(z, a.nz);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int sx = src_of(L::cx[q], xd, x, xu);
+    const int sy = src_of(L::cy[q], yd, y, yu);
+    const int sz = src_of(L::cz[q], zd, z, zu);
+    const typename Store<S>::T* base = uniform_ptr(fin + int64_t(q) * a.isq + int64_t(sz) * a.isz);
+    r[q] = ld_off(base, uint32_t(sy) * uint32_t(a.isy) + uint32_t(sx));
   }
 }
 
@@ -727,13 +767,134 @@ __global__ __launch_bounds__((tb_threads<TX, TY>())) void k_lbm2(const KArgs<typ
   lbm2_body<L, S, COLL, FEAT, BCK, TX, TY>(a, zc, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// The learned-closure variants hold the matrix-core tiles next to the Q
-// populations; the occupancy floor keeps enough waves in flight to cover HBM
-// latency.
+// The learned-closure step.  Its collision is long (the network runs on the
+// matrix cores beside ~280 VALU instructions per wave), and at the 4 waves per
+// SIMD its registers allow, a wave that loads, collides and stores one cell
+// leaves the SIMD idle while its own loads are in flight.  So a workgroup
+// marches `zper` planes of one tile: the next plane's pulled populations (and
+// solid byte) are loaded while the current plane is collided, and the network's
+// operand blob is staged once into LDS, whose reads count on the LDS counter
+// and so never wait behind the prefetch as global reads of the blob would.
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
+__device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& a, int zper) {
+  static_assert(STREAM, "the learned closure runs in the fused step");
+  using ST = Store<S>;
+  using T = typename ST::T;
+  using C = typename ST::C;
+  constexpr int Q = L::Q;
+  constexpr bool kAll = (FEAT & FEAT_BC) != 0;
+  __shared__ mlp::Packed s_blob;
+  {
+    const uint32_t* src = static_cast<const uint32_t*>(a.mlp);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&s_blob);
+    const int t = int(threadIdx.y * blockDim.x + threadIdx.x);
+    for (int i = t; i < int(sizeof(mlp::Packed) / 4); i += int(blockDim.x * blockDim.y)) dst[i] = src[i];
+  }
+  __syncthreads();
+
+  const uint32_t tile = blockIdx.x;
+  const uint32_t txi = tile % uint32_t(a.tiles_x);
+  const uint32_t rest = tile / uint32_t(a.tiles_x);
+  const uint32_t tyi = rest % uint32_t(a.tiles_y);
+  const int z0 = int(rest / uint32_t(a.tiles_y)) * zper;
+  const int z1 = min(z0 + zper, a.nz);
+  const int x = int(txi * blockDim.x + threadIdx.x);
+  const int y = int(tyi * blockDim.y + threadIdx.y);
+  const bool active = x < a.nx && y < a.ny;
+  // Lanes of a ragged tile past the grid's edge work on the last in-grid cell of
+  // their row / column and store the same bits as the lane that owns it, so every
+  // store is unmasked: the compiler merges its wait counts over the paths into the
+  // loop head, and a path with masked or skipped stores makes that head wait for
+  // the previous plane's stores as well as for the prefetch.  Only the force
+  // sample, a sum, is taken from the owning lanes alone.
+  const int xx = min(x, a.nx - 1);
+  const int yy = min(y, a.ny - 1);
+  const T* fin = static_cast<const T*>(a.fin);
+  const int8_t* solid_mask = kAll ? a.solid : nullptr;
+
+  T pre[Q];
+  int pre_mask = 0;
+  auto prefetch = [&](int z, int px, int py) {
+    gather_raw<L, S>(pre, fin, a, px, py, z);
+    if (solid_mask != nullptr)
+      pre_mask = solid_mask[(uint32_t(z) * uint32_t(a.ny) + uint32_t(py)) * uint32_t(a.nx) + uint32_t(px)];
+  };
+  prefetch(z0, xx, yy);
+  // the lane's store offset within a plane, fixed for the march (computed in the
+  // loop, its 32-bit multiply-add became a 64-bit one whose unused high operand
+  // register could be a prefetch destination, i.e. a wait for the prefetch)
+  uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
+  asm volatile("" : "+v"(off));
+  // the loop head then waits only for the prefetch, with the previous plane's
+  // stores still in flight
+  wait_vm_loads();
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  bool any_solid = false;
+  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L, C, true>(fin, a, q, px, py, pz); };
+  auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
+  for (int z = z0; z < z1; ++z) {
+    // Nothing per lane is carried across planes but the prefetched values: the
+    // blob's LDS reads stay inside the loop (the memory clobber), and the lane's
+    // coordinates are re-derived from an opaque copy, so the Q gather offsets
+    // and the boundary's pull offsets are recomputed per plane (a few VALU
+    // instructions) instead of held live across the march, where they spilled.
+    asm volatile("" ::: "memory");
+    int xl = xx, yl = yy;
+    asm volatile("" : "+v"(xl), "+v"(yl));
+    C f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(pre[q], q);
+    const bool solid = pre_mask != 0;
+    any_solid |= solid;
+    const uint32_t cell = (uint32_t(z) * uint32_t(a.ny) + uint32_t(yl)) * uint32_t(a.nx) + uint32_t(xl);
+    // the next plane's loads go out after the boundary's own pulls (inlet /
+    // outlet lanes), so waiting for those never waits for the prefetch
+    auto next = [&] {
+      if (z + 1 < z1) prefetch(z + 1, xl, yl);
+    };
+    update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK>(f, a, xl, yl, z, cell, solid, false, fx, fy, fz, pulled,
+                                                          eq_unit, (mlp::LdsPacked)(&s_blob), next);
+    T* fout = static_cast<T*>(a.fout);
+    // re-derived per plane, so that the stores keep the scalar-base + 32-bit
+    // offset form (a 64-bit offset hoisted out of the loop costs one 64-bit add
+    // per store)
+    uint32_t offl = off;
+    asm volatile("" : "+v"(offl));
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, offl, ST::template put<L>(f[q], q));
+  }
+  if constexpr ((FEAT & FEAT_FORCE) != 0) {
+    if (a.force != nullptr)
+      force_flush(a.force, active ? fx : 0.0, active ? fy : 0.0, active ? fz : 0.0, active && any_solid);
+  }
+}
+
+// planes marched per workgroup by the learned-closure step
+#ifndef TLBM_MLP_ZPER
+#define TLBM_MLP_ZPER 16
+#endif
+
+// The march pays where the step is bound by instruction issue, i.e. with 16-bit
+// storage (256^3 channel + sphere, fp16 deviations, one box: 43.9 GLUPS one cell
+// per lane, 47.0 marching 8 planes, 47.1 marching 16; on another, 2 and 4 planes
+// were slower than 8).  With fp32 storage the step moves twice the bytes and one
+// cell per lane was as fast or faster (36.7 against 35.4 / 34.9 / 33.5 for 4 / 8 /
+// 16 planes); fp64 would need 2Q more registers than 4 waves per SIMD allow.
+template <int S>
+constexpr bool mlp_marches() {
+  return S == ST_F16 || S == ST_BF16 || S == ST_F16_DEV || S == ST_BF16_DEV;
+}
+
 template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TLBM_MLP_WAVES))) void k_lbm_mlp(
-    const KArgs<typename Store<S>::C> a) {
-  lbm_body<L, S, COLL, STREAM, FEAT, BCK>(a);
+    const KArgs<typename Store<S>::C> a, int zper) {
+  if constexpr (mlp_marches<S>()) {
+    lbm_mlp_body<L, S, COLL, STREAM, FEAT, BCK>(a, zper);
+  } else {
+    (void)zper;
+    lbm_body<L, S, COLL, STREAM, FEAT, BCK>(a);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1212,7 +1373,11 @@ int fill_args(KArgs<C>& a, int lattice, const void* fin, const int64_t* is, void
 template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK = BCK_RUNTIME>
 int launch_lbm(const KArgs<typename Store<S>::C>& a, int bx, int by, uint32_t nb, hipStream_t s) {
   if constexpr ((FEAT & FEAT_MLP) != 0) {
-    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT, BCK>), dim3(nb), dim3(bx, by), 0, s, a);
+    // nb = tiles_x * tiles_y * nz; the closure step marches TLBM_MLP_ZPER planes per workgroup
+    const int zper = mlp_marches<S>() ? TLBM_MLP_ZPER : 1;
+    const uint32_t nbz = uint32_t(a.tiles_x) * uint32_t(a.tiles_y) * uint32_t((a.nz + zper - 1) / zper);
+    (void)nb;
+    hipLaunchKernelGGL((k_lbm_mlp<L, S, COLL, STREAM, FEAT, BCK>), dim3(nbz), dim3(bx, by), 0, s, a, zper);
   } else {
     hipLaunchKernelGGL((k_lbm<L, S, COLL, STREAM, FEAT, BCK>), dim3(nb), dim3(bx, by), 0, s, a);
   }
