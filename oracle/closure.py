@@ -283,9 +283,16 @@ def fused_mlp_step(f, tau, params, cs_delta_sq=0.0, emulate_mfma=True, operands=
     """
     from . import ref
 
+    return mlp_collide(ref.stream(f), tau, params, cs_delta_sq, emulate_mfma, operands)
+
+
+def mlp_collide(fin, tau, params, cs_delta_sq=0.0, emulate_mfma=True, operands="fp16"):
+    """The collision half of :func:`fused_mlp_step`, on post-stream populations ``fin`` (after any
+    boundary overrides and bounce-back, as the fused step orders them).  Returns (f_out, ν_t)."""
+    from . import ref
+
     params = np.asarray(params, np.float64)
     vec, mean, istd = params[:N_PARAMS], params[N_PARAMS:N_PARAMS + N_IN], params[N_PARAMS + N_IN:]
-    fin = ref.stream(f)
     feq, rho_safe, _ = ref._feq_safe(fin)
     fneq = fin - feq
     pi = ref.neq_stress(fneq)

@@ -801,12 +801,15 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   const int x = int(txi * blockDim.x + threadIdx.x);
   const int y = int(tyi * blockDim.y + threadIdx.y);
   const bool active = x < a.nx && y < a.ny;
+  // Waves wholly past the grid's edge (tiles wider or taller than the grid) leave;
+  // there is no barrier after this point.
+  if (__all(!active)) return;
   // Lanes of a ragged tile past the grid's edge work on the last in-grid cell of
-  // their row / column and store the same bits as the lane that owns it, so every
-  // store is unmasked: the compiler merges its wait counts over the paths into the
-  // loop head, and a path with masked or skipped stores makes that head wait for
-  // the previous plane's stores as well as for the prefetch.  Only the force
-  // sample, a sum, is taken from the owning lanes alone.
+  // their row and store the same bits as the lane that owns it (which is in the
+  // same wave), so every store is unmasked: the compiler merges its wait counts
+  // over the paths into the loop head, and a path with masked or skipped stores
+  // makes that head wait for the previous plane's stores as well as for the
+  // prefetch.  Only the force sample, a sum, is taken from the owning lanes alone.
   const int xx = min(x, a.nx - 1);
   const int yy = min(y, a.ny - 1);
   const T* fin = static_cast<const T*>(a.fin);
@@ -830,7 +833,19 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   wait_vm_loads();
   double fx = 0.0, fy = 0.0, fz = 0.0;
   bool any_solid = false;
-  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L, C, true>(fin, a, q, px, py, pz); };
+  // The inlet (x = 0) and outlet (x = nx - 1) lanes need the pulled state of
+  // x = 1 and x = nx - 2, which a lane of the same wave has just gathered.  Before
+  // the update such a lane takes that state into its own f with a lane permute,
+  // and `pulled` hands back f itself (the inlet sums it before overwriting f, the
+  // outlet copies it onto itself), instead of Q more loads and their address
+  // arithmetic.  The inlet's neighbour is always lane 1 of its wave (tiles and
+  // waves start at multiples of 64 in x); the outlet's is in the same wave unless
+  // nx - 1 is a multiple of 64, and is then loaded.  Clamped lanes past the edge
+  // act as outlet lanes too, hence a fixed source lane, not lane - 1.  With a
+  // force buffer the loads stay: the force sample reads a solid edge cell's own
+  // populations before the boundary.
+  constexpr bool kForceFeat = (FEAT & FEAT_FORCE) != 0;
+  const bool out_shfl = !kForceFeat && ((a.nx - 1) & 63) != 0;
   auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
   for (int z = z0; z < z1; ++z) {
     // Nothing per lane is carried across planes but the prefetched values: the
@@ -846,6 +861,22 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(pre[q], q);
     const bool solid = pre_mask != 0;
     any_solid |= solid;
+    if constexpr (kAll && !kForceFeat) {
+      const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
+      const bool take = xl == 0 || (out_shfl && xl == a.nx - 1);
+      if ((bck == TLBM_BC_CHANNEL_X || bck == TLBM_BC_FAR_FIELD) && __any(take)) {
+        const int src = xl == 0 ? 1 : ((a.nx - 2) & 63);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const C g = __shfl(f[q], src, 64);
+          f[q] = take ? g : f[q];
+        }
+      }
+    }
+    auto pulled = [&](int q, int px, int py, int pz) -> C {
+      if (!kForceFeat && (px == 1 || out_shfl)) return f[q];
+      return pull_one<ST, L, C, true>(fin, a, q, px, py, pz);
+    };
     const uint32_t cell = (uint32_t(z) * uint32_t(a.ny) + uint32_t(yl)) * uint32_t(a.nx) + uint32_t(xl);
     // the next plane's loads go out after the boundary's own pulls (inlet /
     // outlet lanes), so waiting for those never waits for the prefetch

@@ -270,3 +270,61 @@ def test_learned_closure_improves_les_over_no_closure(dev):
     # velocity error against the filtered DNS, averaged over the run (the whole pipeline is
     # deterministic: fixed-order reductions, seeded data)
     assert e_mlp < 0.97 * e_none
+
+
+# ---------------------------------------------------------------------------
+# 16-bit storage: the closure step marches z with the next plane prefetched
+# ---------------------------------------------------------------------------
+
+FAR = {"far_field_faces": ["y-", "y+"], "periodic_faces": ["z-", "z+"]}
+# (nz, ny, nx), boundary: chunks of 16 planes with a remainder (19, 17, 18); ragged 64/128/256-wide
+# tiles whose lanes past nx are clamped (70, 40, 33, 130); the outlet's neighbour in the same wave
+# (taken by lane permute) or, at nx = 65, in the previous wave (loaded)
+MARCH_CASES = [((19, 6, 70), "channel_x"), ((5, 4, 65), "channel_x"), ((17, 3, 130), "far_field"),
+               ((3, 5, 40), "channel_x"), ((18, 4, 33), None)]
+
+
+def _ulp(v, mant_bits):
+    return np.exp2(np.floor(np.log2(np.maximum(np.abs(v), 2.0 ** -14))) - mant_bits)
+
+
+@pytest.mark.parametrize("shape,boundary", MARCH_CASES, ids=lambda c: str(c))
+@pytest.mark.parametrize("dtype,operands", [(torch.float16, "fp16"), (torch.bfloat16, "bf16")])
+def test_closure_march_matches_oracle_chain(dev, shape, boundary, dtype, operands):
+    """Every cell of one fused closure step in 16-bit deviation storage against the oracle chain
+    stream -> inlet / outlet / far field -> bounce-back -> closure collision (MFMA emulated): two
+    units in the last place of the stored deviation, except where an fp16 rounding of a layer-1
+    activation flips between the hardware and the emulation (a few cells in 10^4, ν_t then moves by
+    ~1e-3 relative).  A wrong neighbour, plane or lane would miss on a whole plane or row."""
+    nz, ny, nx = shape
+    w = ref.D3Q19.w.reshape(19, 1, 1, 1)
+    f0 = F.random_state(19, shape, seed=40 + nx, noise=2e-2, u_amp=0.05)
+    st = torch.from_numpy(f0 - w).to(dtype)
+    f_abs = st.double().numpy() + w
+    solid = ref.sphere_mask(nx, ny, nz, nx / 3.0, ny / 2.0, nz / 2.0, max(1.2, min(ny, nz) / 2.5))
+    assert solid.any() and not solid.all()
+    tau, u_in, uy, uz = 0.56, 0.04, 0.01, -0.005
+    fin = ref.stream(f_abs)
+    if boundary == "channel_x":
+        fb = ref.channel_bc(fin, u_in, solid=solid)
+    elif boundary == "far_field":
+        fb = ref.far_field_bc(fin, u_in, solid, uy=uy, uz=uz, bc_config=FAR)
+    else:
+        fb = ref.bounce_back(fin, solid)
+    feq, rho_s, _ = ref._feq_safe(fb)
+    x = oc.features_from_pi(ref.neq_stress(fb - feq), rho_s, tau).reshape(-1, 7)
+    m = _model(dev, 23, x.mean(0), x.std(0), scale=1.5, operands=operands)
+    exp_f, exp_nu = oc.mlp_collide(fb, tau, m.params.cpu().double().numpy(), emulate_mfma=True, operands=operands)
+    assert np.ptp(exp_nu) > 1e-2
+    solver = tl.FusedSolver3D(nz, ny, nx, tau, dev, dtype, deviation=True, closure=m,
+                              solid=torch.from_numpy(solid), boundary=boundary,
+                              u_bc=(u_in, uy, uz) if boundary == "far_field" else (u_in, 0.0, 0.0),
+                              bc_config=FAR if boundary == "far_field" else None, layout="contiguous")
+    out = solver.step(st.to(dev)).cpu().double().numpy() + w
+    tol = 2.0 * _ulp(exp_f - w, 10 if dtype == torch.float16 else 7) + 2e-6
+    err = np.abs(out - exp_f)
+    bad = err > tol
+    assert bad.mean() < 1e-3, (int(bad.sum()), np.argwhere(bad)[:5].tolist())
+    assert err.max() < 3e-4, float(err.max())
+    for plane in (bad[..., 0], bad[..., -1], bad[:, 0], bad[:, -1]):  # inlet, outlet, first and last z
+        assert plane.mean() < 0.02, plane.mean()
