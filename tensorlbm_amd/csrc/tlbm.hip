@@ -430,6 +430,17 @@ __device__ __forceinline__ void force_flush(double* force, double fx, double fy,
   }
 }
 
+// Whether the lane at x takes, before the update, the pulled state of the cell
+// its boundary reads (x = 1 for the channel inlet, x = nx - 2 for a copy outlet),
+// so that `pulled` can hand back its own f: only lanes whose boundary reads it.
+// The far-field inlet and lateral faces set an equilibrium and read nothing; a
+// periodic x face keeps its own state.
+__device__ __forceinline__ bool take_neighbour(int bck, int faces, int x, int nx, bool out_shfl) {
+  if (bck == TLBM_BC_CHANNEL_X) return x == 0 || (out_shfl && x == nx - 1);
+  if (bck == TLBM_BC_FAR_FIELD) return out_shfl && x == nx - 1 && !(faces & TLBM_FACE_XP_PERIODIC);
+  return false;
+}
+
 template <class L, int S, int COLL, bool STREAM, int FEAT, int BCK>
 __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   using ST = Store<S>;
@@ -469,7 +480,26 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
   }
   const bool solid = mask != 0;
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  auto pulled = [&](int q, int px, int py, int pz) { return pull_one<ST, L>(fin, a, q, px, py, pz); };
+  // inlet / outlet neighbours by lane permute, as in lbm_mlp_body (inactive lanes
+  // take part in the permute; what they compute is never stored)
+  constexpr bool kForceFeat = (FEAT & FEAT_FORCE) != 0;
+  const bool out_shfl = !kForceFeat && ((a.nx - 1) & 63) != 0;
+  if constexpr (kAll && !kForceFeat && STREAM) {
+    const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
+    const bool take = take_neighbour(bck, a.bc_faces, xx, a.nx, out_shfl);
+    if (__any(take)) {
+      const int src = xx == 0 ? 1 : ((a.nx - 2) & 63);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const C g = __shfl(f[q], src, 64);
+        f[q] = take ? g : f[q];
+      }
+    }
+  }
+  auto pulled = [&](int q, int px, int py, int pz) -> C {
+    if (!kForceFeat && (px == 1 || out_shfl)) return f[q];
+    return pull_one<ST, L>(fin, a, q, px, py, pz);
+  };
   auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
   update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK>(f, a, xx, yy, z, cell, solid, false, fx, fy, fz, pulled,
                                                         eq_unit);
@@ -863,8 +893,8 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     any_solid |= solid;
     if constexpr (kAll && !kForceFeat) {
       const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
-      const bool take = xl == 0 || (out_shfl && xl == a.nx - 1);
-      if ((bck == TLBM_BC_CHANNEL_X || bck == TLBM_BC_FAR_FIELD) && __any(take)) {
+      const bool take = take_neighbour(bck, a.bc_faces, xl, a.nx, out_shfl);
+      if (__any(take)) {
         const int src = xl == 0 ? 1 : ((a.nx - 2) & 63);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {

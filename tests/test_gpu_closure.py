@@ -277,11 +277,16 @@ def test_learned_closure_improves_les_over_no_closure(dev):
 # ---------------------------------------------------------------------------
 
 FAR = {"far_field_faces": ["y-", "y+"], "periodic_faces": ["z-", "z+"]}
+# x faces periodic: the outlet keeps its own state (and so does the x = 0 plane on the far field's
+# x- periodic form)
+FAR_CONFIGS = {"far_field": FAR, "far_field_xper": {"far_field_faces": ["y-", "z+"], "periodic_faces": ["x+"]},
+               "far_field_xper2": {"far_field_faces": ["y+", "z-"], "periodic_faces": ["x-", "z-"]}}
 # (nz, ny, nx), boundary: chunks of 16 planes with a remainder (19, 17, 18); ragged 64/128/256-wide
 # tiles whose lanes past nx are clamped (70, 40, 33, 130); the outlet's neighbour in the same wave
 # (taken by lane permute) or, at nx = 65, in the previous wave (loaded)
 MARCH_CASES = [((19, 6, 70), "channel_x"), ((5, 4, 65), "channel_x"), ((17, 3, 130), "far_field"),
-               ((3, 5, 40), "channel_x"), ((18, 4, 33), None)]
+               ((3, 5, 40), "channel_x"), ((18, 4, 33), None), ((7, 5, 70), "far_field_xper"),
+               ((7, 5, 70), "far_field_xper2")]
 
 
 def _ulp(v, mant_bits):
@@ -307,8 +312,8 @@ def test_closure_march_matches_oracle_chain(dev, shape, boundary, dtype, operand
     fin = ref.stream(f_abs)
     if boundary == "channel_x":
         fb = ref.channel_bc(fin, u_in, solid=solid)
-    elif boundary == "far_field":
-        fb = ref.far_field_bc(fin, u_in, solid, uy=uy, uz=uz, bc_config=FAR)
+    elif boundary in FAR_CONFIGS:
+        fb = ref.far_field_bc(fin, u_in, solid, uy=uy, uz=uz, bc_config=FAR_CONFIGS[boundary])
     else:
         fb = ref.bounce_back(fin, solid)
     feq, rho_s, _ = ref._feq_safe(fb)
@@ -316,10 +321,11 @@ def test_closure_march_matches_oracle_chain(dev, shape, boundary, dtype, operand
     m = _model(dev, 23, x.mean(0), x.std(0), scale=1.5, operands=operands)
     exp_f, exp_nu = oc.mlp_collide(fb, tau, m.params.cpu().double().numpy(), emulate_mfma=True, operands=operands)
     assert np.ptp(exp_nu) > 1e-2
+    far = boundary in FAR_CONFIGS
     solver = tl.FusedSolver3D(nz, ny, nx, tau, dev, dtype, deviation=True, closure=m,
-                              solid=torch.from_numpy(solid), boundary=boundary,
-                              u_bc=(u_in, uy, uz) if boundary == "far_field" else (u_in, 0.0, 0.0),
-                              bc_config=FAR if boundary == "far_field" else None, layout="contiguous")
+                              solid=torch.from_numpy(solid), boundary="far_field" if far else boundary,
+                              u_bc=(u_in, uy, uz) if far else (u_in, 0.0, 0.0),
+                              bc_config=FAR_CONFIGS[boundary] if far else None, layout="contiguous")
     out = solver.step(st.to(dev)).cpu().double().numpy() + w
     tol = 2.0 * _ulp(exp_f - w, 10 if dtype == torch.float16 else 7) + 2e-6
     err = np.abs(out - exp_f)
