@@ -530,11 +530,18 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t w, uint32_t n) {
 // before this plane's step-two stores, i.e. a wait until at most Q operations
 // remain.  The compiler takes the strictest wait over the paths that reach the
 // loop head, and the paths that issue no stores (the march's entry, the waves
-// and fill planes without step-two work) would force vmcnt(0), i.e. waiting for
-// the stores to reach memory as well.  Those paths wait for their loads here
-// instead, so the merged wait at the loop head becomes vmcnt(Q).
-// Encoding: vmcnt [3:0] and [15:14] = 0, expcnt [6:4] = 7, lgkmcnt [11:8] = 15.
-__device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// without step-two work) would force vmcnt(0), i.e. waiting for the stores to
+// reach memory as well.  Those paths wait for their prefetched loads here, with
+// an empty asm that reads the destination registers: the compiler places the
+// wait itself and knows that nothing is pending on that path afterwards.  (An
+// explicit s_waitcnt is outside its model: it kept the loads pending after one,
+// and the loop head's merged wait was vmcnt(0).)
+template <class V, int N>
+__device__ __forceinline__ void settle_loads(const V (&pre)[N], const int& mask) {
+#pragma unroll
+  for (int q = 0; q < N; ++q) asm volatile("" ::"v"(pre[q]));
+  asm volatile("" ::"v"(mask));
+}
 
 // LDS writes complete, then the workgroup barrier.  Unlike __syncthreads() this
 // does not wait for outstanding global loads, so the next plane's prefetch
@@ -721,7 +728,7 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
     prefetch(p1 + 1);  // p1 + 1 <= z0 + 1 <= zend
     lds_barrier();
   }
-  wait_vm_loads();
+  settle_loads(pre, pre_mask);
   for (int k = 0; z0 + k < zend; ++k) {
     const int z = z0 + k;
     const int p1 = z + 1;
@@ -773,7 +780,7 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
         }
       }
     } else {
-      wait_vm_loads();
+      settle_loads(pre, pre_mask);
     }
   }
   if constexpr (kForce) {
@@ -860,7 +867,7 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   asm volatile("" : "+v"(off));
   // the loop head then waits only for the prefetch, with the previous plane's
   // stores still in flight
-  wait_vm_loads();
+  settle_loads(pre, pre_mask);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   bool any_solid = false;
   // The inlet (x = 0) and outlet (x = nx - 1) lanes need the pulled state of
