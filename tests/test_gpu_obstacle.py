@@ -141,3 +141,41 @@ def test_fused_far_field_matches_oracle_chain(dev, bc_config, dtype):
         want = fin - (fin - feq) / tau
     tol = dict(rtol=1e-5, atol=1e-7) if dtype == torch.float32 else dict(rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(f.cpu().numpy(), want, **tol)
+
+
+# The one-step kernel's inlet (x = 0) and outlet (x = nx - 1) lanes take the pulled state of x = 1 /
+# x = nx - 2 from the lane of their wave that gathered it.  nx - 1 a multiple of 64 puts the outlet's
+# neighbour in the previous wave (loaded instead); lanes past nx (ragged 64/128/256-wide tiles) act
+# as outlet lanes; x-periodic far-field faces keep their own state.
+EDGE_NX = [33, 65, 70, 129, 130, 200]
+FAR_CFG = {"channel_x": None, "far_field": {"far_field_faces": ["y-", "y+"], "periodic_faces": ["z-", "z+"]},
+           "far_field_xper": {"far_field_faces": ["y-", "z+"], "periodic_faces": ["x+"]},
+           "far_field_xper2": {"far_field_faces": ["y+"], "periodic_faces": ["x-", "z-", "z+"]}}
+
+
+@pytest.mark.parametrize("nx", EDGE_NX)
+@pytest.mark.parametrize("boundary", list(FAR_CFG))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_one_step_inlet_outlet_lanes(dev, nx, boundary, dtype):
+    ny, nz, tau = 3, 4, 0.56
+    uin = (0.05, 0.01, -0.02)
+    solid = ref.sphere_mask(nx, ny, nz, nx / 3.0, 1.0, 2.0, 1.3)
+    f0 = F.random_state(19, (nz, ny, nx), seed=nx, noise=2e-2, u_amp=0.05)
+    far = boundary.startswith("far")
+    s = tl.FusedSolver3D(nz, ny, nx, tau, dev, dtype, solid=torch.tensor(solid, device=dev),
+                         boundary="far_field" if far else "channel_x", u_bc=uin if far else (uin[0], 0.0, 0.0),
+                         bc_config=FAR_CFG[boundary], layout="contiguous", temporal_blocking=False)
+    got = s.step(_gpu(f0, dev, dtype)).cpu().numpy()
+    f_in = f0.astype(np.float32).astype(np.float64) if dtype == torch.float32 else f0  # what the GPU was given
+    fin = ref.stream(f_in)
+    if far:
+        fb = ref.far_field_bc(fin, uin[0], solid, uin[1], uin[2], FAR_CFG[boundary])
+    else:
+        fb = ref.channel_bc(fin, uin[0], solid=solid)
+    feq, _, _ = ref._feq_safe(fb)
+    want = fb - (fb - feq) / tau
+    tol = dict(rtol=1e-5, atol=1e-7) if dtype == torch.float32 else dict(rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(got, want, **tol)
+    # the edge planes are where a wrong lane would show
+    np.testing.assert_allclose(got[..., 0], want[..., 0], **tol)
+    np.testing.assert_allclose(got[..., -1], want[..., -1], **tol)
