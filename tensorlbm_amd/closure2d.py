@@ -34,8 +34,8 @@ from . import _native as nat
 
 __all__ = [
     "ModelArch", "EddyViscosityMLP", "save_model", "load_model", "strain_rate_tensor_2d", "extract_les_samples_2d",
-    "EddyViscosityDataset", "save_dataset_pt", "load_dataset_pt", "TrainConfig", "train_eddy_viscosity_model",
-    "predict_nu_t_2d", "predict_tau_eff_2d", "collide_ai_les_bgk",
+    "extract_les_samples_2d_multi", "EddyViscosityDataset", "save_dataset_pt", "load_dataset_pt", "TrainConfig",
+    "train_eddy_viscosity_model", "predict_nu_t_2d", "predict_tau_eff_2d", "collide_ai_les_bgk",
 ]
 
 _ACTIVATIONS = {"tanh": 0, "relu": 1, "gelu": 2}
@@ -261,6 +261,15 @@ def extract_les_samples_2d(ux: torch.Tensor, uy: torch.Tensor, c_s: float = 0.1,
         keep = (~mask.to(f.device, torch.bool).reshape(-1)).nonzero(as_tuple=False).squeeze(-1)
         f, t = f.index_select(0, keep), t.index_select(0, keep)
     return f.contiguous(), t.contiguous()
+
+
+def extract_les_samples_2d_multi(snapshots, c_s: float = 0.1, mask: torch.Tensor | None = None):
+    """Samples of many ``(ux, uy)`` snapshots concatenated; empty (0, 3) / (0, 1) tensors when there are
+    none (dataset.py:130-144)."""
+    rows = [extract_les_samples_2d(ux, uy, c_s=c_s, mask=mask) for ux, uy in snapshots]
+    if not rows:
+        return torch.zeros(0, 3), torch.zeros(0, 1)
+    return torch.cat([r[0] for r in rows], dim=0), torch.cat([r[1] for r in rows], dim=0)
 
 
 @dataclass

@@ -69,3 +69,8 @@ def test_pipeline_refuses_a_cpu_device(tmp_path):
         p2.run_ai_les_pipeline(tmp_path, device="cpu")
     with pytest.raises(ValueError, match="data_source"):
         p2.run_ai_les_pipeline(tmp_path, data_source="rans")
+
+
+def test_multi_snapshot_samples_of_nothing_are_empty():
+    f, t = c2.extract_les_samples_2d_multi([])
+    assert tuple(f.shape) == (0, 3) and tuple(t.shape) == (0, 1)

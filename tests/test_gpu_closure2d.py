@@ -131,6 +131,17 @@ def test_strain_and_samples():
     np.testing.assert_allclose(target.cpu().numpy(), want_t.numpy(), rtol=1e-6, atol=0)
 
 
+def test_multi_snapshot_samples():
+    """dataset.py:130-144: the snapshots' samples in order, the mask applied to each."""
+    snaps = [[v.cuda() for v in _velocity(12, 20, s)] for s in (21, 22, 23)]
+    mask = torch.zeros(12, 20, dtype=torch.bool, device="cuda:0")
+    mask[3:5, 4:9] = True
+    f, t = c2.extract_les_samples_2d_multi(snaps, c_s=0.13, mask=mask)
+    rows = [c2.extract_les_samples_2d(ux, uy, c_s=0.13, mask=mask) for ux, uy in snaps]
+    assert torch.equal(f, torch.cat([r[0] for r in rows])) and torch.equal(t, torch.cat([r[1] for r in rows]))
+    assert f.shape == (3 * (240 - 10), 3)
+
+
 def test_predictions_and_les_collision():
     m = _model(c2.ModelArch(), 8)
     ux, uy = _velocity(20, 36, 9)
