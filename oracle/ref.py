@@ -1040,6 +1040,27 @@ def collide_cumulant_d3q27(f, tau, omega_b=1.0, omega_odd=1.0, omega_even=1.0, C
     return feq + (Mi @ np.stack(m).reshape(27, -1)).reshape((27,) + sh)
 
 
+def far_field_6face(f: np.ndarray, u_in: float, uy: float = 0.0, uz: float = 0.0) -> np.ndarray:
+    """triton_fused_obstacle.py:2386-2471 (apply_far_field_bc_6face): equilibrium (ρ = 1) on x = 0, copy
+    x = nx−2 onto x = nx−1, equilibrium on y = 0 / ny−1 and, only when nz > 4, on z = 0 / nz−1; in that
+    order, out of place here."""
+    lat = _lat(f)
+    one = np.ones((1, 1, 1))
+    feq = equilibrium(lat, one, one * u_in, one * uy, one * uz)[:, 0, 0, 0][:, None, None]
+    f = f.copy()
+    nz, ny, nx = f.shape[1:]
+    f[:, :, :, 0] = feq
+    if nx >= 2:
+        f[:, :, :, -1] = f[:, :, :, -2]
+    f[:, :, 0, :] = feq
+    if ny >= 2:
+        f[:, :, -1, :] = feq
+    if nz > 4:
+        f[:, 0, :, :] = feq
+        f[:, -1, :, :] = feq
+    return f
+
+
 def far_field_bc(f: np.ndarray, u_in: float, obstacle: np.ndarray | None = None, uy: float = 0.0, uz: float = 0.0,
                  bc_config: dict | None = None) -> np.ndarray:
     """boundaries3d.py:446-537: free-stream equilibrium (ρ = 1) on the inlet x = 0 and the selected

@@ -14,6 +14,8 @@ import torch  # noqa: F401  (loads torch's HIP runtime before the library binds 
 from . import lattice
 from .fused import (
     FusedSolver3D,
+    apply_far_field_bc_6face,
+    apply_mass_correction,
     fused_obstacle_step,
     fused_step,
     hip_collide,
@@ -68,7 +70,7 @@ __version__ = "0.1.0"
 __all__ = list(_ops_all) + [
     "EddyViscosityMLP", "collide_ai_les_bgk", "extract_les_samples_2d", "predict_nu_t_2d", "predict_tau_eff_2d",
     "strain_rate_tensor_2d", "train_eddy_viscosity_model", "AIPipelineResult", "run_ai_les_pipeline",
-    "run_ai_dns_pipeline", "lattice", "FusedSolver3D", "fused_obstacle_step", "fused_step", "hip_collide", "hip_fused", "hip_stream",
+    "run_ai_dns_pipeline", "lattice", "FusedSolver3D", "fused_obstacle_step", "apply_far_field_bc_6face", "apply_mass_correction", "fused_step", "hip_collide", "hip_fused", "hip_stream",
     "is_available", "PrecisionPolicy", "cast_to_compute", "cast_to_store", "StepContext", "ThroughputReporter",
     "CallbackReporter", "triton_fused", "triton_stream", "triton_collide", "triton_fused_obstacle_xfar_les",
     "TritonFusedSolver3D", "LBMStepExecutor", "BatchedCavities", "EddyViscosityMLP3D", "TrainConfig3D",

@@ -221,6 +221,72 @@ def fused_obstacle_step(
     return out
 
 
+def apply_far_field_bc_6face(f: torch.Tensor, u_in: float, *, uy: float = 0.0, uz: float = 0.0,
+                             feq_pad_buf: torch.Tensor | None = None, feq_vec: torch.Tensor | None = None) -> None:
+    """The production companion of the fused obstacle step (triton_fused_obstacle.py:2386-2471), in place:
+    free-stream equilibrium (ρ = 1, u = (u_in, uy, uz)) on x = 0, the zero-gradient copy of x = nx − 2 onto
+    x = nx − 1, then the equilibrium on y = 0, y = ny − 1 and, when nz > 4 (below that the z axis stays
+    the step's periodic wrap, the reference's 2-D-extruded mode), on z = 0 and z = nz − 1, in that order.
+
+    ``f`` holds absolute populations, shape ``(Q, nz, ny, nx)`` with Q ≥ 19 (slots past 19 are set to 0,
+    as the reference pads its 32-slot buffers).  ``feq_vec`` (shape ``(19,)`` or ``(Q,)``) skips the
+    equilibrium; otherwise it comes from the library's equilibrium kernel on one cell.  These are
+    O(Q·area) plane writes on the device, as in the reference; ``FusedSolver3D(boundary="far_field")``
+    applies the same faces inside the step instead.
+    """
+    if f.dim() != 4 or f.shape[0] < 19:
+        raise ValueError(f"f must be (Q >= 19, nz, ny, nx), got shape {tuple(f.shape)}")
+    nat.require_gpu(f, "apply_far_field_bc_6face")
+    q, nz, ny, nx = (int(v) for v in f.shape)
+    if feq_vec is None:
+        from .ops import equilibrium3d
+
+        one = torch.ones((1, 1, 1), dtype=torch.float64, device=f.device)
+        feq_vec = equilibrium3d(one, one * float(u_in), one * float(uy), one * float(uz))[:, 0, 0, 0]
+    elif feq_vec.dim() != 1:
+        raise ValueError(f"feq_vec must be 1-D shape (Q,), got shape {tuple(feq_vec.shape)}")
+    feq_vec = feq_vec.to(device=f.device, dtype=f.dtype)
+    if feq_vec.shape[0] < q:
+        if feq_pad_buf is None or feq_pad_buf.shape != (q,) or feq_pad_buf.dtype != f.dtype or \
+                feq_pad_buf.device != f.device:
+            feq_pad_buf = torch.zeros((q,), dtype=f.dtype, device=f.device)
+        else:
+            feq_pad_buf.zero_()
+        feq_pad_buf[: feq_vec.shape[0]] = feq_vec
+        feq_vec = feq_pad_buf
+    elif feq_vec.shape[0] > q:
+        feq_vec = feq_vec[:q]
+    face = feq_vec[:, None, None]
+    f[:, :, :, 0] = face
+    if nx >= 2:
+        f[:, :, :, -1] = f[:, :, :, -2]
+    f[:, :, 0, :] = face
+    if ny >= 2:
+        f[:, :, -1, :] = face
+    if nz > 4:
+        f[:, 0, :, :] = face
+        f[:, -1, :, :] = face
+
+
+def apply_mass_correction(f: torch.Tensor, target_mass: float) -> torch.Tensor:
+    """Rescale ``f`` in place so that its sum is ``target_mass`` and return it
+    (triton_fused_obstacle.py:2513-2527, the rule of correct_mass3d); the current mass is an FP64 sum on
+    the device (the reference sums in the storage type), and a vanishing mass leaves ``f`` as it is."""
+    q = int(f.shape[0])
+    if f.dim() != 4 or q not in (19, 27):
+        raise ValueError(f"f must be (19 or 27, nz, ny, nx), got shape {tuple(f.shape)}")
+    nat.require_gpu(f, "apply_mass_correction")
+    from .ops import moment_sums
+
+    cur = moment_sums(f)[0:1]
+    scale = torch.where(cur.abs() < 1e-30, torch.ones_like(cur), float(target_mass) / cur)
+    nz, ny, nx = (int(v) for v in f.shape[1:])
+    rc = nat.lib().tlbm_scale(nat.D3Q19 if q == 19 else nat.D3Q27, nat.store_code(f.dtype), f.data_ptr(),
+                              nat.strides_of(f), nz, ny, nx, scale.data_ptr(), nat.stream_of(f))
+    nat.check(rc, "apply_mass_correction")
+    return f
+
+
 class FusedSolver3D:
     """Ping-pong solver over the fused HIP step (mirror of TritonFusedSolver3D).
 

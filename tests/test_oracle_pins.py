@@ -455,6 +455,29 @@ def test_channel_bc_inlet_outlet():
     assert np.array_equal(out[:, :, :, -1], f[:, :, :, -2])
 
 
+@pytest.mark.parametrize("nz", [3, 4, 5, 7])
+def test_far_field_6face_faces_and_order(nz):
+    ny, nx, u = 5, 6, (0.05, 0.01, -0.02)
+    f = F.random_state(19, (nz, ny, nx), seed=20, noise=1e-2)
+    out = ref.far_field_6face(f, *u)
+    face = np.zeros((nz, ny, nx), dtype=bool)
+    face[:, :, 0] = face[:, 0, :] = face[:, -1, :] = True
+    if nz > 4:
+        face[0] = face[-1] = True
+    r, ux, uy, uz = ref.macroscopic(out)
+    for got, want in ((r, 1.0), (ux, u[0]), (uy, u[1]), (uz, u[2])):
+        assert np.allclose(got[face], want, rtol=0, atol=1e-15)
+    outlet = ~face
+    outlet[:, :, :-1] = False  # the copied column, where no face later overwrites it
+    src = np.roll(outlet, -1, axis=2)
+    assert outlet.sum() == (nz - (2 if nz > 4 else 0)) * (ny - 2)
+    assert np.array_equal(out[:, outlet], f[:, src])
+    rest = ~face & ~outlet
+    assert np.array_equal(out[:, rest], f[:, rest])
+    # y faces come after the copy: the outlet corner holds the equilibrium, not the copied column
+    assert not np.allclose(f[:, 1, 0, -2], out[:, 1, 0, -1])
+
+
 def test_zou_he_lid_imposes_velocity():
     f = F.random_state(19, (3, 5, 6), seed=18, noise=1e-2)
     out = ref.zou_he_moving_lid(f, 0.1)
