@@ -65,9 +65,16 @@ triton_collide = hip_collide
 triton_fused_obstacle_xfar_les = fused_obstacle_step
 TritonFusedSolver3D = FusedSolver3D
 
+# the reference's top-level velocity-set tables (src/tensorlbm/__init__.py:265-272): D2Q9 C/W/OPPOSITE,
+# D3Q19 C3D/W3D/OPPOSITE3D, D3Q27 C27/W27/OPPOSITE27; ``lattice.D3Q19.W_EXACT64`` etc. hold the fp64 weights
+C, W, OPPOSITE = lattice.D2Q9.C, lattice.D2Q9.W, lattice.D2Q9.OPPOSITE
+C3D, W3D, OPPOSITE3D = lattice.D3Q19.C, lattice.D3Q19.W, lattice.D3Q19.OPPOSITE
+C27, W27, OPPOSITE27 = lattice.D3Q27.C, lattice.D3Q27.W, lattice.D3Q27.OPPOSITE
+
 __version__ = "0.1.0"
 
 __all__ = list(_ops_all) + [
+    "C", "W", "OPPOSITE", "C3D", "W3D", "OPPOSITE3D", "C27", "W27", "OPPOSITE27",
     "EddyViscosityMLP", "collide_ai_les_bgk", "extract_les_samples_2d", "predict_nu_t_2d", "predict_tau_eff_2d",
     "strain_rate_tensor_2d", "train_eddy_viscosity_model", "AIPipelineResult", "run_ai_les_pipeline",
     "run_ai_dns_pipeline", "lattice", "FusedSolver3D", "fused_obstacle_step", "apply_far_field_bc_6face", "apply_mass_correction", "fused_step", "hip_collide", "hip_fused", "hip_stream",

@@ -38,10 +38,10 @@ __all__ = [
     "collide_cascaded_d3q27", "collide_cumulant_d2q9", "collide_cumulant_d3q19", "collide_cumulant_d3q27",
     "collide_cumulant_geier_d3q27",
     "collide_smagorinsky_bgk", "collide_smagorinsky_bgk3d", "collide_smagorinsky_bgk27", "collide_smagorinsky_mrt",
-    "collide_smagorinsky_mrt3d", "stream",
+    "collide_smagorinsky_mrt3d", "collide_smagorinsky_mrt27", "stream",
     "stream3d", "stream3d_roll", "stream27", "stream27_roll", "correct_mass", "correct_mass3d",
     "correct_mass27", "bounce_back_cells", "bounce_back_cells_3d", "bounce_back_cells_27",
-    "compute_obstacle_forces_3d", "moment_sums", "gradient_closure_tau", "collide_bgk_fused", "collide_bgk_matmul",
+    "compute_obstacle_forces_3d", "compute_obstacle_forces_27", "moment_sums", "gradient_closure_tau", "collide_bgk_fused", "collide_bgk_matmul",
     "apply_simple_channel_boundaries_3d", "far_field_bc_3d", "zou_he_moving_lid_3d", "sphere_mask",
 ]
 
@@ -418,6 +418,14 @@ def collide_smagorinsky_mrt3d(f, tau, C_s=0.1, s_e=1.19, s_eps=1.4, s_q=1.2, s_p
     return _collide(f, 19, coll, "collide_smagorinsky_mrt3d")
 
 
+def collide_smagorinsky_mrt27(f, tau, C_s=0.1, s_e=1.19, s_eps=1.4, s_q=1.2, s_pi=None):
+    """D3Q27 MRT whose stress moments 5-9 relax at the per-cell Smagorinsky 1/τ_eff (turbulence.py:497-587)."""
+    s_pi = s_e if s_pi is None else s_pi
+    coll = nat.make_collision(nat.COLL_MRT, _tau(tau, "collide_smagorinsky_mrt27"), tau_mode=nat.TAU_SMAG,
+                              eq_rho_floor=False, cs_delta_sq=float(C_s) ** 2, rates=(s_e, s_eps, s_q, s_pi))
+    return _collide(f, 27, coll, "collide_smagorinsky_mrt27")
+
+
 # ---------------------------------------------------------------------------
 # streaming, bounce-back, reductions
 # ---------------------------------------------------------------------------
@@ -568,6 +576,15 @@ def compute_obstacle_forces_3d(f, obstacle_mask):
     nat.check(rc, "compute_obstacle_forces_3d")
     res = acc.to(f.dtype)
     return res[0], res[1], res[2]
+
+
+def compute_obstacle_forces_27(f, obstacle_mask):
+    """D3Q27 Ladd force ``2 Σ_solid Σ_q c_q f`` (obstacles.py:197-240), FP64 accumulated; the same
+    reduction kernel as :func:`compute_obstacle_forces_3d`."""
+    if f.ndim != 4 or f.shape[0] != 27:
+        raise ValueError(f"compute_obstacle_forces_27: populations must have shape (27, nz, ny, nx), "
+                         f"got {tuple(f.shape)}")
+    return compute_obstacle_forces_3d(f, obstacle_mask)
 
 
 def _correct(f, q, target_mass, what):
