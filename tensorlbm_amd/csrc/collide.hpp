@@ -19,17 +19,41 @@ struct Macro {
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 
-// rho = sum f, u = sum c f / max(rho, 1e-12)   (d3q19.py:165-186)
+// Sums start from -0, the additive identity of IEEE arithmetic (x + -0 = x for
+// every x, +0 included; +0 is not one, since -0 + +0 = +0), so the first add or
+// fma of a sum folds into a move or a multiply.
+template <typename C>
+constexpr C kSum0 = C(-0.0);
+
+// a sign-folded term: acc + c v for c in {-1, 1}
+template <typename C>
+__device__ __forceinline__ C add_signed(C acc, int c, C v) {
+  return c > 0 ? acc + v : acc - v;
+}
+
+template <class L>
+constexpr bool has_z() {
+  for (int q = 0; q < L::Q; ++q)
+    if (L::cz[q] != 0) return true;
+  return false;
+}
+
+// rho = sum f, u = sum c f / max(rho, 1e-12)   (d3q19.py:165-186).  Summed one
+// population at a time in q order: every kernel (one step, two steps per launch,
+// the closure march) then rounds these sums identically.  A sum over opposite
+// pairs (f_q + f_opp, f_q - f_opp) saves a few adds but made the one- and
+// two-step kernels disagree by an ulp in about 1 cell in 5000 of masked grids.
 template <class L, typename C>
 __device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
-  C rho = C(0), jx = C(0), jy = C(0), jz = C(0);
+  C rho = kSum0<C>, jx = kSum0<C>, jy = kSum0<C>, jz = kSum0<C>;
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     rho += f[q];
-    if (L::cx[q] != 0) jx += C(L::cx[q]) * f[q];
-    if (L::cy[q] != 0) jy += C(L::cy[q]) * f[q];
-    if (L::cz[q] != 0) jz += C(L::cz[q]) * f[q];
+    if (L::cx[q] != 0) jx = add_signed(jx, L::cx[q], f[q]);
+    if (L::cy[q] != 0) jy = add_signed(jy, L::cy[q], f[q]);
+    if (L::cz[q] != 0) jz = add_signed(jz, L::cz[q], f[q]);
   }
+  if constexpr (!has_z<L>()) jz = C(0);
   Macro<C> m;
   m.rho = rho;
   m.rho_safe = rho > C(1e-12) ? rho : C(1e-12);
@@ -46,7 +70,7 @@ __device__ __forceinline__ void equilibrium(C rho, C ux, C uy, C uz, C (&feq)[L:
   const C usq15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
-    C cu = C(0);
+    C cu = kSum0<C>;
     if (L::cx[q] != 0) cu += C(L::cx[q]) * ux;
     if (L::cy[q] != 0) cu += C(L::cy[q]) * uy;
     if (L::cz[q] != 0) cu += C(L::cz[q]) * uz;
@@ -68,7 +92,7 @@ __device__ __forceinline__ void bgk_relax(C (&f)[L::Q], C rho, C ux, C uy, C uz,
     if (o == q) {
       f[q] = om1 * f[q] + C(L::w[q]) * base;
     } else if (q < o) {
-      C cu = C(0);
+      C cu = kSum0<C>;
       if (L::cx[q] != 0) cu += C(L::cx[q]) * ux;
       if (L::cy[q] != 0) cu += C(L::cy[q]) * uy;
       if (L::cz[q] != 0) cu += C(L::cz[q]) * uz;
@@ -101,7 +125,7 @@ __device__ __forceinline__ void zou_he_lid(C (&f)[19], C u) {
 template <class L, typename C>
 __device__ __forceinline__ void neq_stress(const C (&fneq)[L::Q], C (&pi)[6]) {
 #pragma unroll
-  for (int k = 0; k < 6; ++k) pi[k] = C(0);
+  for (int k = 0; k < 6; ++k) pi[k] = kSum0<C>;
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     if (L::cx[q] != 0) pi[0] += fneq[q];
@@ -119,16 +143,24 @@ __device__ __forceinline__ void neq_stress(const C (&fneq)[L::Q], C (&pi)[6]) {
 // forming f_eq.
 template <class L, typename C>
 __device__ __forceinline__ void neq_stress_moments(const C (&f)[L::Q], C rho, C ux, C uy, C uz, C (&pi)[6]) {
-  C m[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+  // second moments from the opposite-pair sums f_q + f_opp (c_a c_b is even in c)
+  C m[6] = {kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>};
+  {
+#pragma clang fp contract(off)  // no fusion into the pair sums, whatever the calling kernel
 #pragma unroll
-  for (int q = 0; q < L::Q; ++q) {
-    if (L::cx[q] != 0) m[0] += f[q];
-    if (L::cy[q] != 0) m[1] += f[q];
-    if (L::cz[q] != 0) m[2] += f[q];
-    if (L::cx[q] * L::cy[q] != 0) m[3] += C(L::cx[q] * L::cy[q]) * f[q];
-    if (L::cx[q] * L::cz[q] != 0) m[4] += C(L::cx[q] * L::cz[q]) * f[q];
-    if (L::cy[q] * L::cz[q] != 0) m[5] += C(L::cy[q] * L::cz[q]) * f[q];
+    for (int q = 0; q < L::Q; ++q) {
+      const int o = L::opp[q];
+      if (q >= o) continue;
+      const C sq = f[q] + f[o];
+      if (L::cx[q] != 0) m[0] += sq;
+      if (L::cy[q] != 0) m[1] += sq;
+      if (L::cz[q] != 0) m[2] += sq;
+      if (L::cx[q] * L::cy[q] != 0) m[3] = add_signed(m[3], L::cx[q] * L::cy[q], sq);
+      if (L::cx[q] * L::cz[q] != 0) m[4] = add_signed(m[4], L::cx[q] * L::cz[q], sq);
+      if (L::cy[q] * L::cz[q] != 0) m[5] = add_signed(m[5], L::cy[q] * L::cz[q], sq);
+    }
   }
+  if constexpr (!has_z<L>()) m[2] = m[4] = m[5] = C(0);
   const C r3 = rho * C(1.0 / 3.0);
   pi[0] = m[0] - (r3 + rho * ux * ux);
   pi[1] = m[1] - (r3 + rho * uy * uy);
@@ -258,7 +290,7 @@ __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C*
     for (int i = 0; i < Q; ++i) {
       m[i] = C(0);
       if (mrt_group<L>(i) == MRT_CONSERVED) continue;  // relaxed at rate 0: never formed
-      C acc = C(0);
+      C acc = kSum0<C>;
 #pragma unroll
       for (int k = 0; k < Q; ++k)
         if (MrtTab<L>::T[i][k] != 0.0) acc += C(MrtTab<L>::T[i][k]) * mono[k];
@@ -266,7 +298,7 @@ __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C*
     }
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-      C acc = C(0);
+      C acc = kSum0<C>;
 #pragma unroll
       for (int i = 0; i < Q; ++i)
         if (mrt_group<L>(i) != MRT_CONSERVED && MrtTab<L>::TI[k][i] != 0.0) acc += C(MrtTab<L>::TI[k][i]) * m[i];
@@ -281,7 +313,7 @@ __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C*
   for (int i = 0; i < Q; ++i) {
     m[i] = C(0);
     if (mrt_group<L>(i) == MRT_CONSERVED) continue;  // relaxed at rate 0: never formed
-    C acc = C(0);
+    C acc = kSum0<C>;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       constexpr double zero = 0.0;
@@ -298,7 +330,7 @@ __device__ __forceinline__ void mrt(C (&f)[L::Q], const C (&feq)[L::Q], const C*
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    C acc = C(0);
+    C acc = kSum0<C>;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       if (mrt_group<L>(i) != MRT_CONSERVED && MrtTab<L>::Mi[q][i] != 0.0) acc += C(MrtTab<L>::Mi[q][i]) * m[i];
@@ -552,7 +584,7 @@ __device__ __forceinline__ void moments(const C (&v)[L::Q], C (&m)[L::Q]) {
   }
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    C acc = C(0);
+    C acc = kSum0<C>;
 #pragma unroll
     for (int j = 0; j < L::Q; ++j)
       if (CmTab<L>::M[i][j] != 0.0) acc += C(CmTab<L>::M[i][j]) * v[j];
@@ -572,7 +604,7 @@ __device__ __forceinline__ void populations(const C (&m)[L::Q], C (&v)[L::Q]) {
   }
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
-    C acc = C(0);
+    C acc = kSum0<C>;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i)
       if (CmTab<L>::Mi[q][i] != 0.0) acc += C(CmTab<L>::Mi[q][i]) * m[i];

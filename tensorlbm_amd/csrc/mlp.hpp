@@ -83,9 +83,13 @@ __host__ __device__ constexpr int w2_col(int s, int h, int e) { return 4 * h + (
 
 __device__ __forceinline__ f16x8 as_f16x8(u16x8 v) { return __builtin_bit_cast(f16x8, v); }
 
+// log(1 + e^(beta z)) / beta on the hardware base-2 exponential and logarithm
+// (v_exp_f32 / v_log_f32, about 1 ulp each): 1 + e^(beta z) >= 1 is never
+// denormal, so neither needs the range scaling of the library functions
 __device__ __forceinline__ float softplus(float z) {
   const float bz = BETA * z;
-  return bz > 20.0f ? z : __logf(1.0f + __expf(bz)) * (1.0f / BETA);
+  const float e = __builtin_amdgcn_exp2f(bz * 1.4426950408889634f);
+  return bz > 20.0f ? z : __builtin_amdgcn_logf(1.0f + e) * (0.6931471805599453f / BETA);
 }
 
 // max(v, 0) as one integer max on the bit pattern (negative floats are negative
@@ -125,7 +129,7 @@ __device__ __forceinline__ float block_out(PP P, int h, const f32x16& c, f16x8 a
   for (int r = 0; r < 16; ++r) d[r] = P->b2c[h][r];
   d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a20, act16(c, 0), d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a21, act16(c, 1), d, 0, 0, 0);
-  float p = 0.0f;
+  float p = -0.0f;  // -0: the first fma folds into a multiply
 #pragma unroll
   for (int r = 0; r < 16; ++r) p += P->w3c[h][r] * relu(d[r]);
   return p + __shfl_xor(p, 32, 64);
@@ -193,7 +197,7 @@ __device__ __forceinline__ float block_out_bf16(PP P, int h, const f32x16& c, bf
   act_bf16(c, 1, bh, bl);
   d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, bh, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, bl, d, 0, 0, 0);
-  float p = 0.0f;
+  float p = -0.0f;  // -0: the first fma folds into a multiply
 #pragma unroll
   for (int r = 0; r < 16; ++r) p += P->w3c[h][r] * relu(d[r]);
   return p + __shfl_xor(p, 32, 64);

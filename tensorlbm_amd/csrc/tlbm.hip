@@ -825,11 +825,22 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   constexpr int Q = L::Q;
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
   __shared__ mlp::Packed s_blob;
+  // f_eq(rho = 1, bc_u) for the inlet and the far-field faces, formed once per
+  // workgroup instead of by every wave with such a lane on every plane
+  __shared__ C s_eq[kAll ? Q : 1];
   {
     const uint32_t* src = static_cast<const uint32_t*>(a.mlp);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&s_blob);
     const int t = int(threadIdx.y * blockDim.x + threadIdx.x);
     for (int i = t; i < int(sizeof(mlp::Packed) / 4); i += int(blockDim.x * blockDim.y)) dst[i] = src[i];
+    if constexpr (kAll) {
+      if (t == 0) {
+        C e[Q];
+        equilibrium_here<L>(C(1), a.bc_u, e);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) s_eq[q] = e[q];
+      }
+    }
   }
   __syncthreads();
 
@@ -887,7 +898,10 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   // populations before the boundary.
   constexpr bool kForceFeat = (FEAT & FEAT_FORCE) != 0;
   const bool out_shfl = !kForceFeat && ((a.nx - 1) & 63) != 0;
-  auto eq_unit = [&](C (&e)[Q]) { equilibrium_here<L>(C(1), a.bc_u, e); };
+  auto eq_unit = [&](C (&e)[Q]) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) e[q] = s_eq[q];
+  };
   for (int z = z0; z < z1; ++z) {
     // Nothing per lane is carried across planes but the prefetched values: the
     // blob's LDS reads stay inside the loop (the memory clobber), and the lane's
