@@ -414,74 +414,120 @@ constexpr bool matches_table() {
 }
 static_assert(matches_table(), "D3Q27 moment exponents must follow the CM27 rows");
 
+// Sweep steps that know at compile time which operands are structurally zero
+// (the D3Q19 populations and moments embedded in D3Q27 arrays leave 8 of 27
+// entries zero): an operand flagged zero is dropped instead of added, since
+// x + 0 does not fold for the compiler (it differs from x when x = -0).  The
+// flags are constants once the sweeps are unrolled; the results equal the
+// plain sums except possibly for the sign of an exact zero.
 template <typename C>
-__device__ __forceinline__ void forward(const C (&v)[27], C (&m)[27]) {
+__device__ __forceinline__ C zadd(C a, bool za, C b, bool zb) {
+  return za ? b : (zb ? a : a + b);
+}
+template <typename C>
+__device__ __forceinline__ C zsub(C a, bool za, C b, bool zb) {
+  return zb ? a : (za ? -b : a - b);
+}
+template <typename C>
+__device__ __forceinline__ C zfma(C k, C a, bool za, C b, bool zb) {
+  return za ? b : (zb ? k * a : fma(k, a, b));
+}
+
+// raw moments m[DEG] = sum_q cx^a cy^b cz^c v_q by three 1-D sweeps (x, y, z);
+// zv[q] flags the populations known to be zero
+template <typename C>
+__device__ __forceinline__ void forward(const C (&v)[27], C (&m)[27], const bool (&zv)[27]) {
   C g[3][3][3], h[3][3][3];  // after the x sweep: [a][cy][cz]; after the y sweep: [a][b][cz]
+  bool zg[3][3][3], zh[3][3][3];
 #pragma unroll
   for (int y = 0; y < 3; ++y)
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-      const C vm = v[IDX.pop[0][y][z]], v0 = v[IDX.pop[1][y][z]], vp = v[IDX.pop[2][y][z]];
-      const C sum = vp + vm;
-      g[0][y][z] = sum + v0;
-      g[1][y][z] = vp - vm;
+      const int qm = IDX.pop[0][y][z], q0 = IDX.pop[1][y][z], qp = IDX.pop[2][y][z];
+      const bool zs = zv[qp] && zv[qm];
+      const C sum = zadd(v[qp], zv[qp], v[qm], zv[qm]);
+      g[0][y][z] = zadd(sum, zs, v[q0], zv[q0]);
+      g[1][y][z] = zsub(v[qp], zv[qp], v[qm], zv[qm]);
       g[2][y][z] = sum;
+      zg[0][y][z] = zs && zv[q0];
+      zg[1][y][z] = zg[2][y][z] = zs;
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-      const C vm = g[a][0][z], v0 = g[a][1][z], vp = g[a][2][z];
-      const C sum = vp + vm;
-      h[a][0][z] = sum + v0;
-      h[a][1][z] = vp - vm;
+      const bool zm = zg[a][0][z], z0 = zg[a][1][z], zp = zg[a][2][z], zs = zp && zm;
+      const C sum = zadd(g[a][2][z], zp, g[a][0][z], zm);
+      h[a][0][z] = zadd(sum, zs, g[a][1][z], z0);
+      h[a][1][z] = zsub(g[a][2][z], zp, g[a][0][z], zm);
       h[a][2][z] = sum;
+      zh[a][0][z] = zs && z0;
+      zh[a][1][z] = zh[a][2][z] = zs;
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const C vm = h[a][b][0], v0 = h[a][b][1], vp = h[a][b][2];
-      const C sum = vp + vm;
-      m[IDX.mom[a][b][0]] = sum + v0;
-      m[IDX.mom[a][b][1]] = vp - vm;
+      const bool zm = zh[a][b][0], z0 = zh[a][b][1], zp = zh[a][b][2], zs = zp && zm;
+      const C sum = zadd(h[a][b][2], zp, h[a][b][0], zm);
+      m[IDX.mom[a][b][0]] = zadd(sum, zs, h[a][b][1], z0);
+      m[IDX.mom[a][b][1]] = zsub(h[a][b][2], zp, h[a][b][0], zm);
       m[IDX.mom[a][b][2]] = sum;
     }
 }
 
+// the populations of raw moments, the three sweeps inverted; zm[i] flags the
+// moments known to be zero
 template <typename C>
-__device__ __forceinline__ void inverse(const C (&m)[27], C (&v)[27]) {
+__device__ __forceinline__ void inverse(const C (&m)[27], C (&v)[27], const bool (&zm)[27]) {
   C g[3][3][3], h[3][3][3];  // [a][b][cz], then [a][cy][cz]
+  bool zg[3][3][3], zh[3][3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const C m0 = m[IDX.mom[a][b][0]], m1 = m[IDX.mom[a][b][1]], m2 = m[IDX.mom[a][b][2]];
-      const C half = C(0.5) * m2;
-      h[a][b][1] = m0 - m2;
-      h[a][b][2] = fma(C(0.5), m1, half);
-      h[a][b][0] = fma(C(-0.5), m1, half);
+      const int i0 = IDX.mom[a][b][0], i1 = IDX.mom[a][b][1], i2 = IDX.mom[a][b][2];
+      const C half = C(0.5) * m[i2];
+      h[a][b][1] = zsub(m[i0], zm[i0], m[i2], zm[i2]);
+      h[a][b][2] = zfma(C(0.5), m[i1], zm[i1], half, zm[i2]);
+      h[a][b][0] = zfma(C(-0.5), m[i1], zm[i1], half, zm[i2]);
+      zh[a][b][1] = zm[i0] && zm[i2];
+      zh[a][b][0] = zh[a][b][2] = zm[i1] && zm[i2];
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-      const C m0 = h[a][0][z], m1 = h[a][1][z], m2 = h[a][2][z];
-      const C half = C(0.5) * m2;
-      g[a][1][z] = m0 - m2;
-      g[a][2][z] = fma(C(0.5), m1, half);
-      g[a][0][z] = fma(C(-0.5), m1, half);
+      const bool z0 = zh[a][0][z], z1 = zh[a][1][z], z2 = zh[a][2][z];
+      const C half = C(0.5) * h[a][2][z];
+      g[a][1][z] = zsub(h[a][0][z], z0, h[a][2][z], z2);
+      g[a][2][z] = zfma(C(0.5), h[a][1][z], z1, half, z2);
+      g[a][0][z] = zfma(C(-0.5), h[a][1][z], z1, half, z2);
+      zg[a][1][z] = z0 && z2;
+      zg[a][0][z] = zg[a][2][z] = z1 && z2;
     }
 #pragma unroll
   for (int y = 0; y < 3; ++y)
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-      const C m0 = g[0][y][z], m1 = g[1][y][z], m2 = g[2][y][z];
-      const C half = C(0.5) * m2;
-      v[IDX.pop[1][y][z]] = m0 - m2;
-      v[IDX.pop[2][y][z]] = fma(C(0.5), m1, half);
-      v[IDX.pop[0][y][z]] = fma(C(-0.5), m1, half);
+      const bool z0 = zg[0][y][z], z1 = zg[1][y][z], z2 = zg[2][y][z];
+      const C half = C(0.5) * g[2][y][z];
+      v[IDX.pop[1][y][z]] = zsub(g[0][y][z], z0, g[2][y][z], z2);
+      v[IDX.pop[2][y][z]] = zfma(C(0.5), g[1][y][z], z1, half, z2);
+      v[IDX.pop[0][y][z]] = zfma(C(-0.5), g[1][y][z], z1, half, z2);
     }
+}
+
+constexpr bool kNone27[27] = {};
+
+template <typename C>
+__device__ __forceinline__ void forward(const C (&v)[27], C (&m)[27]) {
+  forward(v, m, kNone27);
+}
+
+template <typename C>
+__device__ __forceinline__ void inverse(const C (&m)[27], C (&v)[27]) {
+  inverse(m, v, kNone27);
 }
 // Raw moments of the second-order equilibrium w_q rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2)
 // in closed form.  D3Q27's weights are products of 1-D weights (2/3, 1/6, 1/6) whose
@@ -547,6 +593,20 @@ constexpr bool matches_table19() {
 }
 static_assert(matches_table19(), "D3Q19 moment exponents must follow the CM19 rows");
 
+// the D3Q27 entries a D3Q19 distribution leaves zero: the 8 corner populations
+// and the 8 moments whose three exponents are all nonzero
+struct Zero19 {
+  bool pop[27];
+  bool mom[27];
+};
+constexpr Zero19 make_zero19() {
+  Zero19 t{};
+  for (int q = 0; q < 27; ++q) t.pop[q] = D3Q27::cx[q] != 0 && D3Q27::cy[q] != 0 && D3Q27::cz[q] != 0;
+  for (int i = 0; i < 27; ++i) t.mom[i] = DEG[i][0] != 0 && DEG[i][1] != 0 && DEG[i][2] != 0;
+  return t;
+}
+constexpr Zero19 Z19 = make_zero19();
+
 template <typename C>
 __device__ __forceinline__ void forward19(const C (&v)[19], C (&m)[19]) {
   C v27[27], m27[27];
@@ -554,7 +614,7 @@ __device__ __forceinline__ void forward19(const C (&v)[19], C (&m)[19]) {
   for (int q = 0; q < 27; ++q) v27[q] = C(0);
 #pragma unroll
   for (int q = 0; q < 19; ++q) v27[IDX.pop[D3Q19::cx[q] + 1][D3Q19::cy[q] + 1][D3Q19::cz[q] + 1]] = v[q];
-  forward(v27, m27);
+  forward(v27, m27, Z19.pop);
 #pragma unroll
   for (int i = 0; i < 19; ++i) m[i] = m27[IDX.mom[DEG19[i][0]][DEG19[i][1]][DEG19[i][2]]];
 }
@@ -566,7 +626,7 @@ __device__ __forceinline__ void inverse19(const C (&m)[19], C (&v)[19]) {
   for (int i = 0; i < 27; ++i) m27[i] = C(0);
 #pragma unroll
   for (int i = 0; i < 19; ++i) m27[IDX.mom[DEG19[i][0]][DEG19[i][1]][DEG19[i][2]]] = m[i];
-  inverse(m27, v27);
+  inverse(m27, v27, Z19.mom);
 #pragma unroll
   for (int q = 0; q < 19; ++q) v[q] = v27[IDX.pop[D3Q19::cx[q] + 1][D3Q19::cy[q] + 1][D3Q19::cz[q] + 1]];
 }
