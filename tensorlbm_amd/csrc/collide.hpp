@@ -81,11 +81,14 @@ __device__ __forceinline__ void equilibrium(C rho, C ux, C uy, C uz, C (&feq)[L:
 // BGK relaxation f <- (1-w) f + w f_eq with the equilibrium evaluated per
 // opposite pair: f_eq(q), f_eq(opp q) = s +/- a with s = w_q rho (1 - 1.5u^2 + 4.5 cu^2)
 // and a = 3 w_q rho cu (same algebra as d3q19.py:119-121, fewer operations).
-template <class L, typename C>
+// kDevOut returns f - w_q instead, the deviation storage's value: s - w_q =
+// w_q (base - 1) + ..., one subtraction per cell in place of one per population.
+template <class L, typename C, bool kDevOut = false>
 __device__ __forceinline__ void bgk_relax(C (&f)[L::Q], C rho, C ux, C uy, C uz, C omega) {
   const C om1 = C(1) - omega;
   const C orho = omega * rho;
-  const C base = orho * (C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz));
+  const C base0 = orho * (C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz));
+  const C base = kDevOut ? base0 - C(1) : base0;
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     const int o = L::opp[q];

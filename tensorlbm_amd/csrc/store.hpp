@@ -30,6 +30,7 @@ struct Store<ST_F32> {
   __device__ static __forceinline__ C get(T v, int) { return v; }
   template <class L>
   __device__ static __forceinline__ T put(C v, int) { return v; }
+  static constexpr bool kDev = false;
 };
 
 template <>
@@ -40,6 +41,7 @@ struct Store<ST_F64> {
   __device__ static __forceinline__ C get(T v, int) { return v; }
   template <class L>
   __device__ static __forceinline__ T put(C v, int) { return v; }
+  static constexpr bool kDev = false;
 };
 
 template <>
@@ -50,6 +52,7 @@ struct Store<ST_F16> {
   __device__ static __forceinline__ C get(T v, int) { return float(v); }
   template <class L>
   __device__ static __forceinline__ T put(C v, int) { return _Float16(v); }
+  static constexpr bool kDev = false;
 };
 
 template <>
@@ -60,6 +63,7 @@ struct Store<ST_BF16> {
   __device__ static __forceinline__ C get(T v, int) { return bf16_to_f32(v); }
   template <class L>
   __device__ static __forceinline__ T put(C v, int) { return f32_to_bf16(v); }
+  static constexpr bool kDev = false;
 };
 
 template <>
@@ -70,6 +74,15 @@ struct Store<ST_F16_DEV> {
   __device__ static __forceinline__ C get(T v, int q) { return float(v) + float(L::w[q]); }
   template <class L>
   __device__ static __forceinline__ T put(C v, int q) { return _Float16(v - float(L::w[q])); }
+  static constexpr bool kDev = true;
+  // v already holds f - w (the BGK relaxation's deviation output).  The empty asm
+  // keeps the conversion a v_cvt_f16_f32 of the fp32 value: the relaxation's last
+  // fma and the conversion would otherwise fuse into one v_fma_mixlo_f16 (a single
+  // rounding) in some kernels and not in others, and the kernels would disagree.
+  __device__ static __forceinline__ T put_dev(C v) {
+    asm("" : "+v"(v));
+    return _Float16(v);
+  }
 };
 
 template <>
@@ -80,6 +93,8 @@ struct Store<ST_BF16_DEV> {
   __device__ static __forceinline__ C get(T v, int q) { return bf16_to_f32(v) + float(L::w[q]); }
   template <class L>
   __device__ static __forceinline__ T put(C v, int q) { return f32_to_bf16(v - float(L::w[q])); }
+  static constexpr bool kDev = true;
+  __device__ static __forceinline__ T put_dev(C v) { return f32_to_bf16(v); }
 };
 
 }  // namespace tlbm

@@ -194,6 +194,19 @@ __device__ __forceinline__ void own_force_sample(const KArgs<typename Store<S>::
 // staged it (an LDS pointer), else from the global blob a.mlp.
 // `before_collision()` runs after the boundary loads and bounce-back, before the
 // collision (the closure kernel issues its next plane's loads there).
+// The BGK collision of the deviation storage forms hands back f - w (the
+// relaxation absorbs the shift), so their stores convert only; every other
+// collision, and streaming alone, hands back f.
+template <int S, int COLL>
+constexpr bool dev_out() {
+  return Store<S>::kDev && COLL == TLBM_COLL_BGK;
+}
+template <class L, int S, int COLL>
+__device__ __forceinline__ typename Store<S>::T put_out(typename Store<S>::C v, int q) {
+  if constexpr (dev_out<S, COLL>()) return Store<S>::put_dev(v);
+  else return Store<S>::template put<L>(v, q);
+}
+
 struct NoHook {
   __device__ void operator()() const {}
 };
@@ -317,7 +330,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
-      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, approx_rcp(te));
+      bgk_relax<L, C, dev_out<S, COLL>()>(f, rho_eq, mc.ux, mc.uy, mc.uz, approx_rcp(te));
     } else if constexpr (COLL == TLBM_COLL_BGK) {
       if (need_pi) {
         C feq[Q], fneq[Q], pi[6];
@@ -334,7 +347,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       if constexpr (!STREAM) {
         if (a.tau_mode == TLBM_TAU_FIELD64) omega = C(1) / C(reinterpret_cast<const double*>(a.tau_eff)[cell]);
       }
-      bgk_relax<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, omega);
+      bgk_relax<L, C, dev_out<S, COLL>()>(f, rho_eq, mc.ux, mc.uy, mc.uz, omega);
     } else {
       // the Geier collision takes the equilibrium's moments in closed form and the
       // LES stress from the moments of f, so it never forms f_eq
@@ -507,7 +520,7 @@ __device__ __forceinline__ void lbm_body(const KArgs<typename Store<S>::C>& a) {
     T* fout = static_cast<T*>(a.fout);
     const uint32_t off = uint32_t(yy) * uint32_t(a.osy) + uint32_t(xx);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, ST::template put<L>(f[q], q));
+    for (int q = 0; q < Q; ++q) st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, off, put_out<L, S, COLL>(f[q], q));
   }
   if constexpr ((FEAT & FEAT_FORCE) != 0) {
     if (a.force != nullptr) force_flush(a.force, fx, fy, fz, solid);
@@ -705,7 +718,7 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int row = (L::cz[q] < 0 ? row_a : L::cz[q] == 0 ? row_b : row_c) + R::index(q);
-      s[row * EC + t] = ST::template put<L>(f[q], q);
+      s[row * EC + t] = put_out<L, S, COLL>(f[q], q);
     }
     if constexpr (kAll) s_solid[(j % 3) * EC + t] = solid ? 1 : 0;
   };
@@ -779,8 +792,8 @@ __device__ __forceinline__ void lbm2_body(const KArgs<typename Store<S>::C>& a, 
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           T* const base = fout + int64_t(q) * a.osq + int64_t(z) * a.osz;
-          if constexpr (BoxWalls) st_l2(base, off, ST::template put<L>(f[q], q));
-          else st_nt(base, off, ST::template put<L>(f[q], q));
+          if constexpr (BoxWalls) st_l2(base, off, put_out<L, S, COLL>(f[q], q));
+          else st_nt(base, off, put_out<L, S, COLL>(f[q], q));
         }
       }
     } else {
@@ -948,7 +961,7 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     asm volatile("" : "+v"(offl));
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-      st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, offl, ST::template put<L>(f[q], q));
+      st_nt(fout + int64_t(q) * a.osq + int64_t(z) * a.osz, offl, put_out<L, S, COLL>(f[q], q));
   }
   if constexpr ((FEAT & FEAT_FORCE) != 0) {
     if (a.force != nullptr)
