@@ -275,16 +275,31 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
         for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
       }
     }
-    // full-way bounce-back; the wave vote skips the (if-converted) swaps in the
-    // waves that hold no solid cell, i.e. almost all of them
+    // full-way bounce-back; the wave vote skips the swaps in the waves that hold
+    // no solid cell, i.e. almost all of them.  As per-lane selects the swapped
+    // values stay in the registers of the unswapped ones, so the waves that skip
+    // pay no copies at the join (a lane-masked exchange left 18 register copies
+    // on every wave's path of the closure step); the force kernels and the plain
+    // 16-bit deviation steps measured faster with the exchange.
     if (__any(solid)) {
-      if (solid) {
+      if constexpr (kForce || (Store<S>::kDev && !kMlp)) {
+        if (solid) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            if (q < L::opp[q]) {
+              const C t = f[q];
+              f[q] = f[L::opp[q]];
+              f[L::opp[q]] = t;
+            }
+          }
+        }
+      } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           if (q < L::opp[q]) {
-            const C t = f[q];
-            f[q] = f[L::opp[q]];
-            f[L::opp[q]] = t;
+            const C a = f[q], b = f[L::opp[q]];
+            f[q] = solid ? b : a;
+            f[L::opp[q]] = solid ? a : b;
           }
         }
       }
