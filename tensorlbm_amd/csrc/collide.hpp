@@ -43,7 +43,9 @@ constexpr bool has_z() {
 // the closure march) then rounds these sums identically.  A sum over opposite
 // pairs (f_q + f_opp, f_q - f_opp) saves a few adds but made the one- and
 // two-step kernels disagree by an ulp in about 1 cell in 5000 of masked grids.
-template <class L, typename C>
+// kDevIn: f holds deviations f - w_q, whose sum is rho - 1 (the weights sum to 1
+// and their first moments vanish, so the momentum sums are unchanged).
+template <class L, typename C, bool kDevIn = false>
 __device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
   C rho = kSum0<C>, jx = kSum0<C>, jy = kSum0<C>, jz = kSum0<C>;
 #pragma unroll
@@ -54,6 +56,7 @@ __device__ __forceinline__ Macro<C> macro(const C (&f)[L::Q]) {
     if (L::cz[q] != 0) jz = add_signed(jz, L::cz[q], f[q]);
   }
   if constexpr (!has_z<L>()) jz = C(0);
+  if constexpr (kDevIn) rho += C(1);
   Macro<C> m;
   m.rho = rho;
   m.rho_safe = rho > C(1e-12) ? rho : C(1e-12);
@@ -83,12 +86,15 @@ __device__ __forceinline__ void equilibrium(C rho, C ux, C uy, C uz, C (&feq)[L:
 // and a = 3 w_q rho cu (same algebra as d3q19.py:119-121, fewer operations).
 // kDevOut returns f - w_q instead, the deviation storage's value: s - w_q =
 // w_q (base - 1) + ..., one subtraction per cell in place of one per population.
-template <class L, typename C, bool kDevOut = false>
+// kDevIn (with kDevOut) takes f - w_q as well: (1 - omega) w_q joins the same
+// term, w_q (base - omega).
+template <class L, typename C, bool kDevOut = false, bool kDevIn = false>
 __device__ __forceinline__ void bgk_relax(C (&f)[L::Q], C rho, C ux, C uy, C uz, C omega) {
+  static_assert(kDevOut || !kDevIn, "deviations in, deviations out");
   const C om1 = C(1) - omega;
   const C orho = omega * rho;
   const C base0 = orho * (C(1) - C(1.5) * (ux * ux + uy * uy + uz * uz));
-  const C base = kDevOut ? base0 - C(1) : base0;
+  const C base = kDevIn ? base0 - omega : (kDevOut ? base0 - C(1) : base0);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     const int o = L::opp[q];
@@ -144,7 +150,9 @@ __device__ __forceinline__ void neq_stress(const C (&fneq)[L::Q], C (&pi)[6]) {
 // the second-order equilibrium are exact on lattices with 4th-order isotropic
 // weights (D2Q9, D3Q19, D3Q27), so this equals neq_stress(f - f_eq) without
 // forming f_eq.
-template <class L, typename C>
+// kDevIn: f holds deviations f - w_q, whose diagonal second moments lack the
+// weights' 1/3, so rho / 3 becomes (rho - 1) / 3.
+template <class L, typename C, bool kDevIn = false>
 __device__ __forceinline__ void neq_stress_moments(const C (&f)[L::Q], C rho, C ux, C uy, C uz, C (&pi)[6]) {
   // second moments from the opposite-pair sums f_q + f_opp (c_a c_b is even in c)
   C m[6] = {kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>, kSum0<C>};
@@ -164,7 +172,7 @@ __device__ __forceinline__ void neq_stress_moments(const C (&f)[L::Q], C rho, C 
     }
   }
   if constexpr (!has_z<L>()) m[2] = m[4] = m[5] = C(0);
-  const C r3 = rho * C(1.0 / 3.0);
+  const C r3 = (kDevIn ? rho - C(1) : rho) * C(1.0 / 3.0);
   pi[0] = m[0] - (r3 + rho * ux * ux);
   pi[1] = m[1] - (r3 + rho * uy * uy);
   pi[2] = m[2] - (r3 + rho * uz * uz);

@@ -75,6 +75,8 @@ struct Store<ST_F16_DEV> {
   template <class L>
   __device__ static __forceinline__ T put(C v, int q) { return _Float16(v - float(L::w[q])); }
   static constexpr bool kDev = true;
+  // the stored deviation itself, for a step that collides in deviations
+  __device__ static __forceinline__ C get_dev(T v) { return float(v); }
   // v already holds f - w (the BGK relaxation's deviation output).  The empty asm
   // keeps the conversion a v_cvt_f16_f32 of the fp32 value: the relaxation's last
   // fma and the conversion would otherwise fuse into one v_fma_mixlo_f16 (a single
@@ -94,6 +96,7 @@ struct Store<ST_BF16_DEV> {
   template <class L>
   __device__ static __forceinline__ T put(C v, int q) { return f32_to_bf16(v - float(L::w[q])); }
   static constexpr bool kDev = true;
+  __device__ static __forceinline__ C get_dev(T v) { return bf16_to_f32(v); }
   __device__ static __forceinline__ T put_dev(C v) { return f32_to_bf16(v); }
 };
 

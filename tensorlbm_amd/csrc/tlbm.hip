@@ -210,8 +210,10 @@ __device__ __forceinline__ typename Store<S>::T put_out(typename Store<S>::C v, 
 struct NoHook {
   __device__ void operator()() const {}
 };
-template <class L, int S, int COLL, bool STREAM, int FEAT, int BCN, int BCK, class Pulled, class EqUnit,
-          class Blob = const mlp::Packed*, class Hook = NoHook>
+// kDevIn: f (and what `pulled` returns) holds the deviations f - w_q of a 16-bit
+// deviation store through to the collision; the closure march's BGK step only.
+template <class L, int S, int COLL, bool STREAM, int FEAT, int BCN, int BCK, bool kDevIn = false, class Pulled,
+          class EqUnit, class Blob = const mlp::Packed*, class Hook = NoHook>
 __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], const KArgs<typename Store<S>::C>& a,
                                             int xx, int yy, int z, uint32_t cell, bool solid, bool nb_edge,
                                             double& fx, double& fy, double& fz, const Pulled& pulled,
@@ -222,6 +224,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
   constexpr bool kForce = (FEAT & FEAT_FORCE) != 0;
   constexpr bool kMlp = (FEAT & FEAT_MLP) != 0;
+  static_assert(!kDevIn || (dev_out<S, COLL>() && kMlp), "deviations through the closure's BGK step only");
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   // a compile-time kind folds the other boundaries away
   const int bck = BCK == BCK_RUNTIME ? a.bc_kind : BCK;
@@ -250,9 +253,18 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
 #pragma unroll
           for (int q = 0; q < Q; ++q) rho1 += f[q];
         }
-        eq_unit(f);
+        if constexpr (kDevIn) {
+          // rho1 is the neighbour's rho - 1 here: rho_1 e_q - w_q = e_q rho1 + (e_q - w_q)
+          C e[Q], ed[Q];
+          eq_unit(e);
+          eq_unit.dev(ed);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] *= rho1;
+          for (int q = 0; q < Q; ++q) f[q] = fma(e[q], rho1, ed[q]);
+        } else {
+          eq_unit(f);
+#pragma unroll
+          for (int q = 0; q < Q; ++q) f[q] *= rho1;
+        }
       } else if (BCN == BCN_PULL && xx == nx - 1) {
         // zero-gradient outlet: copy the pulled state of x = nx-2
 #pragma unroll
@@ -260,7 +272,17 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       }
     } else if (STREAM && bck == TLBM_BC_LID_Y && yy == ny - 1 && !solid) {
       // Zou-He on the fluid cells of the lid plane; solid cells there keep full-way bounce-back
-      if constexpr (Q == 19) zou_he_lid(f, a.bc_u[0]);
+      if constexpr (Q == 19) {
+        if constexpr (kDevIn) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) f[q] += C(L::w[q]);
+        }
+        zou_he_lid(f, a.bc_u[0]);
+        if constexpr (kDevIn) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q) f[q] -= C(L::w[q]);
+        }
+      }
     } else if (STREAM && bck == TLBM_BC_FAR_FIELD) {
       // far_field_bc_3d (boundaries3d.py:446-537): free-stream equilibrium on the
       // inlet and the selected lateral faces (lateral faces win on edges),
@@ -269,7 +291,8 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       const bool lateral = (yy == 0 && (fm & TLBM_FACE_YM)) || (yy == ny - 1 && (fm & TLBM_FACE_YP)) ||
                            (z == 0 && (fm & TLBM_FACE_ZM)) || (z == nz - 1 && (fm & TLBM_FACE_ZP));
       if (lateral || (xx == 0 && !(fm & TLBM_FACE_XM_PERIODIC))) {
-        eq_unit(f);
+        if constexpr (kDevIn) eq_unit.dev(f);
+        else eq_unit(f);
       } else if (BCN == BCN_PULL && xx == nx - 1 && !(fm & TLBM_FACE_XP_PERIODIC)) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = pulled(q, nx - 2, yy, z);
@@ -308,7 +331,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
   before_collision();
 
   if constexpr (COLL != TLBM_COLL_NONE) {
-    const Macro<C> mc = macro<L>(f);
+    const Macro<C> mc = macro<L, C, kDevIn>(f);
     const C rho_eq = a.eq_floor ? mc.rho_safe : mc.rho;
     C omega = a.omega;
     const bool need_pi = (COLL == TLBM_COLL_CUMULANT && Q == 19) || a.tau_mode == TLBM_TAU_SMAG ||
@@ -317,7 +340,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       // learned closure: strain from the non-equilibrium stress with the
       // Smagorinsky tau as reference time scale, S = -Pi / (2 rho cs^2 tau_ref)
       C pi[6];
-      neq_stress_moments<L>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
+      neq_stress_moments<L, C, kDevIn>(f, rho_eq, mc.ux, mc.uy, mc.uz, pi);
       // features feed an fp16 network: the clamped Smagorinsky time scale (smagorinsky_tau)
       // and the strain scale take hardware reciprocals and square roots (1 ulp)
       const C pn = approx_sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
@@ -345,7 +368,7 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[L::Q], con
       // tau_eff = tau + 3 nu_t, floored at 0.5 + 1e-3 (ai/inference.py:56-65)
       C te = a.tau + C(3) * C(nu);
       te = te < C(0.501) ? C(0.501) : te;
-      bgk_relax<L, C, dev_out<S, COLL>()>(f, rho_eq, mc.ux, mc.uy, mc.uz, approx_rcp(te));
+      bgk_relax<L, C, dev_out<S, COLL>(), kDevIn>(f, rho_eq, mc.ux, mc.uy, mc.uz, approx_rcp(te));
     } else if constexpr (COLL == TLBM_COLL_BGK) {
       if (need_pi) {
         C feq[Q], fneq[Q], pi[6];
@@ -852,10 +875,15 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   using C = typename ST::C;
   constexpr int Q = L::Q;
   constexpr bool kAll = (FEAT & FEAT_BC) != 0;
+  // with 16-bit deviation storage the step runs in f - w from load to store: the
+  // loads skip adding w_q and the collision folds it into one term per cell
+  constexpr bool kDevIn = dev_out<S, COLL>() && (FEAT & FEAT_MLP) != 0;
   __shared__ mlp::Packed s_blob;
   // f_eq(rho = 1, bc_u) for the inlet and the far-field faces, formed once per
-  // workgroup instead of by every wave with such a lane on every plane
+  // workgroup instead of by every wave with such a lane on every plane, and its
+  // deviation f_eq - w for the step in deviations
   __shared__ C s_eq[kAll ? Q : 1];
+  __shared__ C s_eqd[kAll && kDevIn ? Q : 1];
   {
     const uint32_t* src = static_cast<const uint32_t*>(a.mlp);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&s_blob);
@@ -866,7 +894,10 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
         C e[Q];
         equilibrium_here<L>(C(1), a.bc_u, e);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) s_eq[q] = e[q];
+        for (int q = 0; q < Q; ++q) {
+          s_eq[q] = e[q];
+          if constexpr (kDevIn) s_eqd[q] = e[q] - C(L::w[q]);
+        }
       }
     }
   }
@@ -926,10 +957,19 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
   // populations before the boundary.
   constexpr bool kForceFeat = (FEAT & FEAT_FORCE) != 0;
   const bool out_shfl = !kForceFeat && ((a.nx - 1) & 63) != 0;
-  auto eq_unit = [&](C (&e)[Q]) {
+  struct EqTables {
+    const C* e;
+    const C* ed;
+    __device__ void operator()(C (&o)[Q]) const {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) e[q] = s_eq[q];
+      for (int q = 0; q < Q; ++q) o[q] = e[q];
+    }
+    __device__ void dev(C (&o)[Q]) const {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) o[q] = ed[q];
+    }
   };
+  const EqTables eq_unit{s_eq, s_eqd};
   for (int z = z0; z < z1; ++z) {
     // Nothing per lane is carried across planes but the prefetched values: the
     // blob's LDS reads stay inside the loop (the memory clobber), and the lane's
@@ -941,7 +981,10 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     asm volatile("" : "+v"(xl), "+v"(yl));
     C f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = ST::template get<L>(pre[q], q);
+    for (int q = 0; q < Q; ++q) {
+      if constexpr (kDevIn) f[q] = ST::get_dev(pre[q]);
+      else f[q] = ST::template get<L>(pre[q], q);
+    }
     const bool solid = pre_mask != 0;
     any_solid |= solid;
     if constexpr (kAll && !kForceFeat) {
@@ -958,7 +1001,8 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     }
     auto pulled = [&](int q, int px, int py, int pz) -> C {
       if (!kForceFeat && (px == 1 || out_shfl)) return f[q];
-      return pull_one<ST, L, C, true>(fin, a, q, px, py, pz);
+      const C v = pull_one<ST, L, C, true>(fin, a, q, px, py, pz);
+      return kDevIn ? v - C(L::w[q]) : v;
     };
     const uint32_t cell = (uint32_t(z) * uint32_t(a.ny) + uint32_t(yl)) * uint32_t(a.nx) + uint32_t(xl);
     // the next plane's loads go out after the boundary's own pulls (inlet /
@@ -966,8 +1010,8 @@ __device__ __forceinline__ void lbm_mlp_body(const KArgs<typename Store<S>::C>& 
     auto next = [&] {
       if (z + 1 < z1) prefetch(z + 1, xl, yl);
     };
-    update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK>(f, a, xl, yl, z, cell, solid, false, fx, fy, fz, pulled,
-                                                          eq_unit, (mlp::LdsPacked)(&s_blob), next);
+    update_cell<L, S, COLL, STREAM, FEAT, BCN_PULL, BCK, kDevIn>(f, a, xl, yl, z, cell, solid, false, fx, fy, fz,
+                                                                  pulled, eq_unit, (mlp::LdsPacked)(&s_blob), next);
     T* fout = static_cast<T*>(a.fout);
     // re-derived per plane, so that the stores keep the scalar-base + 32-bit
     // offset form (a 64-bit offset hoisted out of the loop costs one 64-bit add
