@@ -283,10 +283,11 @@ FAR_CONFIGS = {"far_field": FAR, "far_field_xper": {"far_field_faces": ["y-", "z
                "far_field_xper2": {"far_field_faces": ["y+", "z-"], "periodic_faces": ["x-", "z-"]}}
 # (nz, ny, nx), boundary: chunks of 16 planes with a remainder (19, 17, 18); ragged 64/128/256-wide
 # tiles whose lanes past nx are clamped (70, 40, 33, 130); the outlet's neighbour in the same wave
-# (taken by lane permute) or, at nx = 65, in the previous wave (loaded)
+# (taken by lane permute) or, at nx = 65, in the previous wave (loaded); a lid whose plane
+# holds solid cells
 MARCH_CASES = [((19, 6, 70), "channel_x"), ((5, 4, 65), "channel_x"), ((17, 3, 130), "far_field"),
                ((3, 5, 40), "channel_x"), ((18, 4, 33), None), ((7, 5, 70), "far_field_xper"),
-               ((7, 5, 70), "far_field_xper2")]
+               ((7, 5, 70), "far_field_xper2"), ((9, 6, 40), "lid_y")]
 
 
 def _ulp(v, mant_bits):
@@ -314,6 +315,8 @@ def test_closure_march_matches_oracle_chain(dev, shape, boundary, dtype, operand
         fb = ref.channel_bc(fin, u_in, solid=solid)
     elif boundary in FAR_CONFIGS:
         fb = ref.far_field_bc(fin, u_in, solid, uy=uy, uz=uz, bc_config=FAR_CONFIGS[boundary])
+    elif boundary == "lid_y":  # Zou-He on the lid's fluid cells; its solid cells bounce back
+        fb = ref.bounce_back(np.where(solid[None], fin, ref.zou_he_moving_lid(fin, u_in)), solid)
     else:
         fb = ref.bounce_back(fin, solid)
     feq, rho_s, _ = ref._feq_safe(fb)
@@ -332,5 +335,5 @@ def test_closure_march_matches_oracle_chain(dev, shape, boundary, dtype, operand
     bad = err > tol
     assert bad.mean() < 1e-3, (int(bad.sum()), np.argwhere(bad)[:5].tolist())
     assert err.max() < 3e-4, float(err.max())
-    for plane in (bad[..., 0], bad[..., -1], bad[:, 0], bad[:, -1]):  # inlet, outlet, first and last z
+    for plane in (bad[..., 0], bad[..., -1], bad[:, 0], bad[:, -1], bad[:, :, -1]):  # x, z and lid faces
         assert plane.mean() < 0.02, plane.mean()
